@@ -1,0 +1,224 @@
+"""ctypes binding of lib/librichgpu.so (include/richgpu.h): the hand-written
+HIP step for gfx950.  There is no CPU fallback: every method calls the
+native library, and constructing a context without a GPU raises."""
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+
+RG_FP64 = 0
+RG_MIXED = 1
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("order", ctypes.c_int32), ("bc", ctypes.c_int32), ("slf", ctypes.c_int32),
+                ("delta_v", ctypes.c_double), ("theta", ctypes.c_double),
+                ("delta_p", ctypes.c_double), ("precision", ctypes.c_int32)]
+
+
+class MeshS(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int32), ("n_faces", ctypes.c_int32),
+                ("n_ghosts", ctypes.c_int32)] + [
+        (k, ctypes.c_void_p) for k in (
+            "face_first", "face_second", "v1x", "v1y", "v2x", "v2y", "pt_x", "pt_y", "cm_x",
+            "cm_y", "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig")]
+
+
+class Error(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_int32), ("index", ctypes.c_int32), ("step", ctypes.c_int32),
+                ("phase", ctypes.c_int32), ("message", ctypes.c_char * 96)]
+
+
+class RichGPUError(RuntimeError):
+    """A device-detected error, carrying the reference's message text."""
+
+    def __init__(self, status, err):
+        self.status = status
+        self.code = err.code
+        self.index = err.index
+        self.step = err.step
+        self.phase = err.phase
+        self.text = err.message.decode(errors="replace")
+        super().__init__(f"{self.text} (code {self.code}, index {self.index}, step {self.step}, "
+                         f"phase {self.phase}, status {status})")
+
+
+MESH_FIELDS = ("face_first", "face_second", "v1x", "v1y", "v2x", "v2y", "pt_x", "pt_y", "cm_x",
+               "cm_y", "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig")
+
+
+def lib():
+    L = _ffi.load("librichgpu")
+    if getattr(L, "_typed", False):
+        return L
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    L.rg_create.argtypes = [P(Params), P(vp)]
+    L.rg_destroy.argtypes = [vp]
+    L.rg_destroy.restype = None
+    L.rg_set_stream.argtypes = [vp, vp]
+    L.rg_set_mesh.argtypes = [vp, P(MeshS)]
+    L.rg_set_face_velocity.argtypes = [vp, vp, vp]
+    for n in ("rg_set_cells", "rg_get_cells", "rg_set_extensives", "rg_get_extensives",
+              "rg_set_fluxes", "rg_flux"):
+        getattr(L, n).argtypes = [vp, vp, vp, vp, vp]
+    L.rg_init_extensives.argtypes = [vp]
+    L.rg_time_step.argtypes = [vp, P(ctypes.c_double)]
+    L.rg_reconstruct.argtypes = [vp, vp, vp, vp]
+    L.rg_update_extensives.argtypes = [vp, ctypes.c_double]
+    L.rg_update_cells.argtypes = [vp]
+    L.rg_advance.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.rg_last_error.argtypes = [vp, P(Error)]
+    L.rg_profile_enable.argtypes = [vp, ctypes.c_int32]
+    L.rg_kernel_count.restype = ctypes.c_int32
+    L.rg_kernel_count.argtypes = []
+    L.rg_kernel_name.restype = ctypes.c_char_p
+    L.rg_kernel_name.argtypes = [ctypes.c_int32]
+    L.rg_profile_read.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double)]
+    L.rg_device_bytes.restype = ctypes.c_int64
+    L.rg_device_bytes.argtypes = [vp]
+    L._typed = True
+    return L
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class RichGPU:
+    """One device context: a mesh, the scheme parameters and the state.
+
+    Arrays are numpy SoA: cells (4, N) = density, pressure, vx, vy;
+    extensives and fluxes (4, n) = mass, energy, momentum x, momentum y;
+    slopes (8, N) = d/dx of the four cell fields, then d/dy."""
+
+    def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.3, order=2, slf=True, delta_v=0.2,
+                 theta=0.5, delta_p=0.7, precision=RG_FP64):
+        self._lib = lib()
+        self.mesh = mesh
+        self.N, self.F = mesh.n_cells, mesh.n_faces
+        prm = Params(gamma, cfl, order, mesh.bc, 1 if slf else 0, delta_v, theta, delta_p,
+                     precision)
+        h = ctypes.c_void_p()
+        self._check(self._lib.rg_create(ctypes.byref(prm), ctypes.byref(h)), handle=None)
+        self._h = h
+        keep = []
+        ptrs = []
+        for k in MESH_FIELDS:
+            a = np.ascontiguousarray(getattr(mesh, k))
+            if a.size == 0:
+                a = np.zeros(1, dtype=a.dtype)
+            keep.append(a)
+            ptrs.append(_p(a))
+        ms = MeshS(mesh.n_cells, mesh.n_faces, mesh.n_ghosts, *ptrs)
+        self._check(self._lib.rg_set_mesh(self._h, ctypes.byref(ms)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, handle=0):
+        if rc == 0:
+            return
+        h = self._h if handle == 0 else handle
+        err = Error()
+        if h:
+            self._lib.rg_last_error(h, ctypes.byref(err))
+        raise RichGPUError(rc, err)
+
+    @staticmethod
+    def _rows(a, n, k=4):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (k, n):
+            raise ValueError(f"expected shape {(k, n)}, got {a.shape}")
+        return a
+
+    def set_face_velocity(self, w):
+        if w is None:
+            self._check(self._lib.rg_set_face_velocity(self._h, None, None))
+            return
+        self._wkeep = self._rows(w, self.F, 2)
+        self._check(self._lib.rg_set_face_velocity(self._h, _p(self._wkeep[0]),
+                                                   _p(self._wkeep[1])))
+
+    def set_cells(self, cells):
+        c = self._rows(cells, self.N)
+        self._check(self._lib.rg_set_cells(self._h, *[_p(c[k]) for k in range(4)]))
+
+    def get_cells(self):
+        c = np.empty((4, self.N))
+        self._check(self._lib.rg_get_cells(self._h, *[_p(c[k]) for k in range(4)]))
+        return c
+
+    def set_extensives(self, ext):
+        e = self._rows(ext, self.N)
+        self._check(self._lib.rg_set_extensives(self._h, *[_p(e[k]) for k in range(4)]))
+
+    def get_extensives(self):
+        e = np.empty((4, self.N))
+        self._check(self._lib.rg_get_extensives(self._h, *[_p(e[k]) for k in range(4)]))
+        return e
+
+    def set_fluxes(self, flux):
+        f = self._rows(flux, self.F)
+        self._check(self._lib.rg_set_fluxes(self._h, *[_p(f[k]) for k in range(4)]))
+
+    def init_extensives(self):
+        self._check(self._lib.rg_init_extensives(self._h))
+
+    def time_step(self):
+        dt = ctypes.c_double()
+        self._check(self._lib.rg_time_step(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def reconstruct(self):
+        s = np.empty((8, self.N))
+        L = np.empty((4, self.F))
+        R = np.empty((4, self.F))
+        self._check(self._lib.rg_reconstruct(self._h, _p(s), _p(L), _p(R)))
+        return s, L, R
+
+    def flux(self):
+        f = np.empty((4, self.F))
+        self._check(self._lib.rg_flux(self._h, *[_p(f[k]) for k in range(4)]))
+        return f
+
+    def update_extensives(self, dt):
+        self._check(self._lib.rg_update_extensives(self._h, float(dt)))
+
+    def update_cells(self):
+        self._check(self._lib.rg_update_cells(self._h))
+
+    def advance(self, nsteps, integrator=2, record_dt=True):
+        dts = np.empty(max(nsteps, 1)) if record_dt else None
+        self._check(self._lib.rg_advance(self._h, integrator, nsteps, _p(dts)))
+        return dts[:nsteps] if record_dt else None
+
+    def set_stream(self, stream_ptr):
+        self._check(self._lib.rg_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    def profile(self, on=True):
+        self._check(self._lib.rg_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        """{kernel name: (launches, total ms)} since profile(True)."""
+        out = {}
+        for k in range(self._lib.rg_kernel_count()):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            self._check(self._lib.rg_profile_read(self._h, k, ctypes.byref(n), ctypes.byref(ms)))
+            if n.value:
+                out[self._lib.rg_kernel_name(k).decode()] = (n.value, ms.value)
+        return out
+
+    def device_bytes(self):
+        return self._lib.rg_device_bytes(self._h)

@@ -1,0 +1,114 @@
+"""Flat (structure-of-arrays) Voronoi meshes, as a RICH VoronoiMesh exposes
+them through its Tessellation accessors, built for jittered lattices by
+lib/librichmesh.so (include/richmesh.h)."""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi
+
+RIGID = 0
+PERIODIC = 1
+
+
+class _LatticeParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double),
+                ("jitter", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("bc", ctypes.c_int32), ("flip_seed", ctypes.c_int32)]
+
+
+def _lib():
+    lib = _ffi.load("librichmesh")
+    if not getattr(lib, "_typed", False):
+        lib.rm_lattice_build.restype = ctypes.c_void_p
+        lib.rm_lattice_build.argtypes = [ctypes.POINTER(_LatticeParams)]
+        lib.rm_free.argtypes = [ctypes.c_void_p]
+        lib.rm_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
+        lib.rm_export.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 17
+        lib.rm_face_velocity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        lib._typed = True
+    return lib
+
+
+@dataclass
+class Mesh:
+    """Flat mesh.  Ghost points are numbered n_cells .. n_cells+n_ghosts-1."""
+    n_cells: int
+    n_faces: int
+    n_ghosts: int
+    bc: int
+    face_first: np.ndarray
+    face_second: np.ndarray
+    v1x: np.ndarray
+    v1y: np.ndarray
+    v2x: np.ndarray
+    v2y: np.ndarray
+    pt_x: np.ndarray
+    pt_y: np.ndarray
+    cm_x: np.ndarray
+    cm_y: np.ndarray
+    volume: np.ndarray
+    tess_volume: np.ndarray
+    csr_off: np.ndarray
+    csr_face: np.ndarray
+    ghost_orig: np.ndarray
+    lat_i: np.ndarray = None
+    lat_j: np.ndarray = None
+    _handle: object = None
+
+    def orig(self, ids):
+        ids = np.asarray(ids)
+        out = ids.copy()
+        g = ids >= self.n_cells
+        out[g] = self.ghost_orig[ids[g] - self.n_cells]
+        return out
+
+    def face_velocity(self, wx, wy):
+        """Face velocities for point velocities (StationaryBox /
+        PeriodicEdgeVelocities)."""
+        if self._handle is None:
+            raise RuntimeError("face_velocity needs a mesh built by lattice()")
+        fwx = np.empty(self.n_faces)
+        fwy = np.empty(self.n_faces)
+        wx = np.ascontiguousarray(wx, dtype=np.float64)
+        wy = np.ascontiguousarray(wy, dtype=np.float64)
+        _lib().rm_face_velocity(self._handle.value, _ffi.ptr(wx), _ffi.ptr(wy),
+                                _ffi.ptr(fwx), _ffi.ptr(fwy))
+        return fwx, fwy
+
+
+class _Handle:
+    def __init__(self, value):
+        self.value = value
+
+    def __del__(self):
+        try:
+            _lib().rm_free(self.value)
+        except Exception:
+            pass
+
+
+def lattice(nx, ny, box=(0.0, 1.0, 0.0, 1.0), jitter=1e-3, seed=0, bc=RIGID, flip_seed=0):
+    """Jittered nx x ny lattice in box=(x0, x1, y0, y1)."""
+    lib = _lib()
+    p = _LatticeParams(nx, ny, box[0], box[1], box[2], box[3], jitter, seed, bc, flip_seed)
+    h = lib.rm_lattice_build(ctypes.byref(p))
+    if not h:
+        raise ValueError("rm_lattice_build rejected the parameters")
+    handle = _Handle(h)
+    n = [ctypes.c_int32() for _ in range(4)]
+    lib.rm_sizes(h, *[ctypes.byref(x) for x in n])
+    N, F, G, nnz = (x.value for x in n)
+    i32 = lambda k: np.empty(k, dtype=np.int32)
+    f64 = lambda k: np.empty(k, dtype=np.float64)
+    arrs = dict(face_first=i32(F), face_second=i32(F), v1x=f64(F), v1y=f64(F), v2x=f64(F),
+                v2y=f64(F), pt_x=f64(N + G), pt_y=f64(N + G), cm_x=f64(N + G),
+                cm_y=f64(N + G), volume=f64(N), tess_volume=f64(N), csr_off=i32(N + 1),
+                csr_face=i32(nnz), ghost_orig=i32(G), lat_i=i32(N), lat_j=i32(N))
+    lib.rm_export(h, *[_ffi.ptr(arrs[k]) for k in (
+        "face_first", "face_second", "v1x", "v1y", "v2x", "v2y", "pt_x", "pt_y", "cm_x", "cm_y",
+        "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig", "lat_i", "lat_j")])
+    return Mesh(n_cells=N, n_faces=F, n_ghosts=G, bc=bc, _handle=handle, **arrs)

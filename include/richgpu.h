@@ -1,0 +1,169 @@
+/*
+ * richgpu.h -- C ABI of the MI355X (gfx950) finite-volume step of RICH.
+ *
+ * RICH's hdsim driver (source/newtonian/two_dimensional/hdsim2d.hpp:43-274)
+ * calls five plugin objects per stage; each entry point below replaces the
+ * body of one of them on a device-resident mesh and state:
+ *
+ *   TimeStepFunction::operator()       time_step_function.hpp:29-35
+ *       -> rg_time_step                 (SimpleCFL, simple_cfl.cpp:58-71)
+ *   SpatialReconstruction::operator()  spatial_reconstruction.hpp:37-42
+ *       -> rg_reconstruct               (LinearGaussImproved.cpp:536-635)
+ *   FluxCalculator::operator()         flux_calculator_2d.hpp:35-44
+ *       -> rg_flux                      (ModularFluxCalculator
+ *                                        modular_flux_calculator.cpp:39-71,
+ *                                        SimpleFluxCalculator
+ *                                        simple_flux_calculator.cpp:216-244,
+ *                                        Hllc hllc.cpp:97-154)
+ *   ExtensiveUpdater::operator()       extensive_updater.hpp:34-43
+ *       -> rg_update_extensives         (simple_extensive_updater.cpp:11-35)
+ *   CellUpdater::operator()            cell_updater_2d.hpp:34-41
+ *       -> rg_update_cells              (simple_cell_updater.cpp:85-106)
+ *
+ * plus the state set-up hdsim does in its constructor (init_extensives,
+ * hdsim2d.cpp:41-66) and a device-resident loop of whole steps
+ * (hdsim::TimeAdvance hdsim2d.cpp:149-241, TimeAdvance2Heun :323-449) for a
+ * static mesh, which is how throughput is measured.
+ *
+ * Conventions: the library owns device buffers, the caller owns host
+ * buffers; host arrays are structure-of-arrays; every call is ordered on
+ * the context's stream and returns 0 or an RG_E* code.  Device-detected
+ * errors carry the reference's UniversalError text (rg_last_error).
+ * A context is used from one host thread at a time (the reference plugins
+ * hold mutable scratch and are not re-entrant either).
+ */
+#ifndef RICHGPU_H
+#define RICHGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rg_ctx rg_ctx;
+
+/* status codes */
+#define RG_OK 0
+#define RG_E_ARG -1      /* bad argument / call order                */
+#define RG_E_HIP -2      /* HIP runtime error                        */
+#define RG_E_DEVICE 1    /* device-detected error: see rg_last_error */
+
+/* device-detected error codes (UniversalError messages of the reference) */
+#define RG_ERR_NEGATIVE_DENSITY 1     /* simple_cell_updater.cpp:33-34   */
+#define RG_ERR_NEGATIVE_THERMAL 2     /* ideal_gas.cpp:20-21             */
+#define RG_ERR_HLLC_NAN 3             /* hllc.cpp:102-103                */
+#define RG_ERR_INVALID_WAVE_SPEEDS 4  /* hllc.cpp:143-148                */
+#define RG_ERR_SINGULAR_MATRIX 5      /* LinearGaussImproved.cpp:111-119 */
+#define RG_ERR_TOO_MANY_NEIGHBORS 6   /* LinearGaussImproved.cpp:73-79   */
+
+#define RG_BC_RIGID 0     /* RigidWallGenerator + StationaryBox            */
+#define RG_BC_PERIODIC 1  /* PeriodicGhostGenerator + PeriodicEdgeVelocities */
+
+#define RG_FP64 0         /* every operation in FP64, reference order      */
+#define RG_MIXED 1        /* FP32 reconstruction + HLLC, FP64 extensives and
+                             primitive recovery (compensated sums)          */
+
+typedef struct rg_params {
+  double gamma;    /* IdealGas(gamma)                      ideal_gas.cpp:5-6 */
+  double cfl;      /* SimpleCFL(cfl)                       simple_cfl.cpp:6   */
+  int32_t order;   /* 1: SimpleFluxCalculator (piecewise constant)
+                      2: ModularFluxCalculator + LinearGaussImproved          */
+  int32_t bc;      /* RG_BC_RIGID / RG_BC_PERIODIC                            */
+  int32_t slf;     /* LinearGaussImproved slf        LinearGaussImproved.hpp:31-40 */
+  double delta_v;  /* shock detector |div v|*L/c       (default 0.2)         */
+  double theta;    /* shocked-limiter coefficient      (default 0.5)         */
+  double delta_p;  /* shock detector pressure ratio    (default 0.7)         */
+  int32_t precision; /* RG_FP64 or RG_MIXED                                  */
+} rg_params;
+
+/* Flat mesh, as a caller holding a RICH Tessellation would flatten it:
+ * faces = getAllEdges() (Edge.hpp:13-51), ghost ids remapped from
+ * [N+3, TotalPointNumber) to [N, N+G), csr = GetCellEdges(i) in order. */
+typedef struct rg_mesh {
+  int32_t n_cells, n_faces, n_ghosts;
+  const int32_t *face_first, *face_second; /* [F] Edge::neighbors            */
+  const double *v1x, *v1y, *v2x, *v2y;     /* [F] Edge::vertices             */
+  const double *pt_x, *pt_y;               /* [N+G] GetMeshPoint             */
+  const double *cm_x, *cm_y;               /* [N+G] GetCellCM                */
+  const double *volume;                    /* [N] CacheData::volumes         */
+  const double *tess_volume;               /* [N] GetVolume (width, slopes)  */
+  const int32_t *csr_off;                  /* [N+1]                          */
+  const int32_t *csr_face;                 /* [csr_off[N]] GetCellEdges      */
+  const int32_t *ghost_orig;               /* [G] GetOriginalIndex           */
+} rg_mesh;
+
+typedef struct rg_error {
+  int32_t code;     /* RG_ERR_* or 0                                  */
+  int32_t index;    /* cell or face index of the first failure        */
+  int32_t step;     /* step of rg_advance (0 outside it)              */
+  int32_t phase;    /* 0 cfl, 1 slopes, 2 fluxes, 3 cells (+3 stage 2) */
+  char message[96]; /* reference message text                          */
+} rg_error;
+
+/* lifecycle ---------------------------------------------------------- */
+int rg_create(const rg_params* params, rg_ctx** out);
+void rg_destroy(rg_ctx* ctx);
+/* Run on a caller's hipStream_t (NULL: the library's own stream). */
+int rg_set_stream(rg_ctx* ctx, void* hip_stream);
+
+/* mesh and state (host arrays) ------------------------------------- */
+/* Upload a mesh (hdsim holds the Tessellation, hdsim2d.hpp:54). */
+int rg_set_mesh(rg_ctx* ctx, const rg_mesh* mesh);
+/* Face velocities [F] as hdsim passes edge_velocities; NULL,NULL = all zero
+ * (StationaryBox/PeriodicEdgeVelocities on a static mesh). */
+int rg_set_face_velocity(rg_ctx* ctx, const double* wx, const double* wy);
+/* cells_ (ComputationalCell density, pressure, velocity), [N] each. */
+int rg_set_cells(rg_ctx* ctx, const double* d, const double* p, const double* vx,
+                 const double* vy);
+int rg_get_cells(rg_ctx* ctx, double* d, double* p, double* vx, double* vy);
+/* extensives_ (Extensive mass, energy, momentum), [N] each. */
+int rg_set_extensives(rg_ctx* ctx, const double* m, const double* E, const double* Mx,
+                      const double* My);
+int rg_get_extensives(rg_ctx* ctx, double* m, double* E, double* Mx, double* My);
+/* Fluxes [F] (per unit length) for rg_update_extensives, when they come
+ * from a flux calculator outside this library. */
+int rg_set_fluxes(rg_ctx* ctx, const double* fm, const double* fE, const double* fMx,
+                  const double* fMy);
+
+/* plugins ----------------------------------------------------------- */
+/* init_extensives (hdsim2d.cpp:41-66) from the current cells. */
+int rg_init_extensives(rg_ctx* ctx);
+/* SimpleCFL::operator(): dt = cfl * min_i 1/max_faces((c+|v-w|)/R_i). */
+int rg_time_step(rg_ctx* ctx, double* dt);
+/* SpatialReconstruction: slopes [8][N] (d/dx of density, pressure, vx, vy,
+ * then d/dy) and face states res[f].first/.second as left/right [4][F]
+ * (density, pressure, vx, vy).  Any output may be NULL. */
+int rg_reconstruct(rg_ctx* ctx, double* slopes, double* left, double* right);
+/* FluxCalculator: fluxes per unit face length, kept on the device for
+ * rg_update_extensives; copied out when the pointers are non-NULL [F]. */
+int rg_flux(rg_ctx* ctx, double* fm, double* fE, double* fMx, double* fMy);
+/* ExtensiveUpdater: extensives -= / += dt*area*flux over faces. */
+int rg_update_extensives(rg_ctx* ctx, double dt);
+/* CellUpdater: cells from extensives and CacheData volumes. */
+int rg_update_cells(rg_ctx* ctx);
+
+/* device-resident steps on a static mesh ------------------------------ */
+/* integrator 1: hdsim::TimeAdvance; 2: hdsim::TimeAdvance2Heun.  Each step
+ * computes its own dt (SimpleCFL) and adds the ZeroForce source.  dt_out
+ * [nsteps] receives the dts (may be NULL).  Synchronises once at the end. */
+int rg_advance(rg_ctx* ctx, int32_t integrator, int32_t nsteps, double* dt_out);
+
+/* diagnostics -------------------------------------------------------- */
+int rg_last_error(rg_ctx* ctx, rg_error* err);
+/* Per-kernel HIP-event timing of every launch on the context's stream. */
+int rg_profile_enable(rg_ctx* ctx, int32_t on);
+/* Kernel ids 0..rg_kernel_count()-1. */
+int32_t rg_kernel_count(void);
+const char* rg_kernel_name(int32_t id);
+/* Synchronises, then returns launches and summed milliseconds of a kernel
+ * since rg_profile_enable(ctx, 1). */
+int rg_profile_read(rg_ctx* ctx, int32_t kernel_id, int64_t* launches, double* total_ms);
+/* Bytes of device memory the context holds. */
+int64_t rg_device_bytes(rg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

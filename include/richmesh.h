@@ -1,0 +1,86 @@
+/*
+ * richmesh.h -- C ABI of the jittered-lattice Voronoi mesh builder.
+ *
+ * In RICH the Voronoi mesh is built on the host by VoronoiMesh
+ * (source/tessellation/VoronoiMesh.cpp:333-399 Initialise, :250-302 build_v)
+ * and read by the hydro plugins through the Tessellation accessors
+ * (tessellation.hpp:21-224).  The GPU path keeps that split: a caller that
+ * owns a VoronoiMesh flattens it into an rg_mesh (richgpu.h).  This library
+ * produces the same flat arrays for the workloads used to test and measure
+ * the path -- jittered square lattices in a rigid-wall box (SquareBox +
+ * RigidWallGenerator) or a periodic box (PeriodicBox + PeriodicGhostGenerator)
+ * -- following VoronoiMesh's conventions:
+ *   - faces are Voronoi edges between two Delaunay circumcentres; faces shorter
+ *     than 1e-8*sqrt(R1*R2) are dropped            (VoronoiMesh.cpp:287-288)
+ *   - a face is created while visiting the lower-numbered of its two
+ *     triangles; neighbours are that triangle's counter-clockwise edge
+ *                                                  (VoronoiMesh.cpp:267-297)
+ *   - each cell lists its faces in creation order, i.e. ascending face id
+ *                                                  (VoronoiMesh.cpp:291-294)
+ *   - centres of mass by CalcCellCM, Tessellation volume by GetVolume
+ *                                                  (VoronoiMesh.cpp:420-451)
+ *   - update volumes by SlabSymmetry::calcVolume   (physical_geometry.cpp:13-21)
+ *   - ghost centres of mass mirrored (rigid) or shifted (periodic)
+ *                                                  (VoronoiMesh.cpp:359-398)
+ * Ghost points are numbered N..N+G-1 (RICH numbers them from N+3; the three
+ * super-triangle points are not part of the flat mesh).
+ * Cells are numbered in Hilbert order of the lattice (RICH Hilbert-orders the
+ * generating points, VoronoiMesh.cpp:337-338).
+ */
+#ifndef RICHMESH_H
+#define RICHMESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RM_BC_RIGID 0
+#define RM_BC_PERIODIC 1
+
+typedef struct rm_lattice_params {
+  int32_t nx, ny;          /* lattice points along x and y                 */
+  double x0, x1, y0, y1;   /* box [x0,x1] x [y0,y1]                        */
+  double jitter;           /* uniform jitter, fraction of the spacing      */
+  uint64_t seed;           /* jitter seed                                  */
+  int32_t bc;              /* RM_BC_RIGID or RM_BC_PERIODIC                */
+  int32_t flip_seed;       /* 0: neighbours as built; else flip the
+                              (first,second) order of a random half of the
+                              faces, to exercise ghost-first faces         */
+} rm_lattice_params;
+
+typedef struct rm_mesh rm_mesh;
+
+/* Build the mesh.  Returns NULL on bad parameters. */
+rm_mesh* rm_lattice_build(const rm_lattice_params* p);
+void rm_free(rm_mesh* m);
+
+/* Sizes: cells N, faces F, ghosts G, CSR entries nnz. */
+void rm_sizes(const rm_mesh* m, int32_t* n_cells, int32_t* n_faces,
+              int32_t* n_ghosts, int32_t* nnz);
+
+/* Copy the flat arrays out; any pointer may be NULL.
+ * face_first/face_second [F]; v1x,v1y,v2x,v2y [F];
+ * pt_x,pt_y,cm_x,cm_y [N+G]; volume,tess_volume [N];
+ * csr_off [N+1]; csr_face [nnz]; ghost_orig [G];
+ * lat_i, lat_j [N] lattice coordinates of each cell. */
+void rm_export(const rm_mesh* m, int32_t* face_first, int32_t* face_second,
+               double* v1x, double* v1y, double* v2x, double* v2y,
+               double* pt_x, double* pt_y, double* cm_x, double* cm_y,
+               double* volume, double* tess_volume, int32_t* csr_off,
+               int32_t* csr_face, int32_t* ghost_orig, int32_t* lat_i,
+               int32_t* lat_j);
+
+/* Face velocities of a static mesh with given point velocities [N]:
+ * VoronoiMesh::CalcFaceVelocity (VoronoiMesh.cpp:146-151) applied as
+ * StationaryBox (rigid; zero on mirror faces, stationary_box.cpp:20-40) or
+ * PeriodicEdgeVelocities (periodic_edge_velocities.cpp:5-21). */
+void rm_face_velocity(const rm_mesh* m, const double* wx, const double* wy,
+                      double* fwx, double* fwy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
