@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Throughput of the device-resident RICH finite-volume step on one MI355X.
+
+Workload: 2D Kelvin-Helmholtz, 2048 x 1024 jittered lattice (2,097,152
+cells) in a periodic [0,2]x[0,1] box, second order (LinearGaussImproved +
+slope limiter, HLLC, Heun), static mesh, FP64.  One step = one
+hdsim::TimeAdvance2Heun pass (two flux stages) over all cells; inputs are
+resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  `value` is Mcell-updates/s (cells x steps /
+s).  `roofline` is for the kernel with the largest summed time in the timed
+region; `stage_roofline` applies the whole-stage algorithmic byte count
+(180 N + 48 F per cell-stage).  `cpu_baseline` times the CPU restatement of
+the reference step (oracle/, OpenMP) on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "huji-rich_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_alg_bytes(name, N, F, G, nnz, moving=False):
+    """Compulsory bytes of one launch (each datum read or written once, FP64
+    values, int32 indices)."""
+    if name == "k_slopes":
+        # cells 32N, ghost sources 4G, CSR 4(N+1)+8nnz, CM+points 32(N+G),
+        # tess volume 8N, face vertices 32F; slopes out 64N
+        return 32 * N + 4 * G + 4 * (N + 1) + 8 * nnz + 32 * (N + G) + 8 * N + 32 * F + 64 * N
+    if name == "k_flux2":
+        # face pair 8F, vertices 32F, cells 32N, slopes 64N, CM+points
+        # 32(N+G), ghost ids 8G, face velocity 16F if moving; flux out 32F
+        return 8 * F + 32 * F + 32 * N + 64 * N + 32 * (N + G) + 8 * G + (16 * F if moving else 0) + 32 * F
+    if name == "k_update":
+        # extensives in 32N, Heun average 32N/2 (one of two stages), flux 32F,
+        # area 8F, CSR 4(N+1)+4nnz, volumes 16N; extensives + cells out 64N
+        return 32 * N + 16 * N + 32 * F + 8 * F + 4 * (N + 1) + 4 * nnz + 16 * N + 64 * N
+    if name == "k_flux1":
+        return 8 * F + 32 * F + 32 * N + 32 * (N + G) + 32 * F
+    if name == "k_cfl":
+        return 32 * N + 8 * N + 4 * (N + 1)
+    return 0
+
+
+def cpu_baseline(mesh, cells, steps, threads):
+    import pyoracle as po
+    o = po.Oracle(mesh, order=2, nthreads=threads)
+    ext = o.init_extensives(cells)
+    t0 = time.perf_counter()
+    c, e, dts = o.run(cells, ext, steps, integrator=2)
+    dt = time.perf_counter() - t0
+    return c, e, dts, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="Heun steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time without per-kernel HIP events")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}, "
+              f"WORLD_SIZE {world}); the multi-GPU path is a design, not a measured run.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import cases
+    from rich_amd.gpu import RichGPU
+
+    t_setup = time.perf_counter()
+    mesh = cases.kh_mesh(args.nx, args.ny)
+    cells = cases.kh_cells(mesh)
+    N, F, G = mesh.n_cells, mesh.n_faces, mesh.n_ghosts
+    nnz = int(mesh.csr_off[-1])
+    g = RichGPU(mesh, order=2, cfl=0.3)
+    g.set_cells(cells)
+    g.init_extensives()
+    ext0 = g.get_extensives()
+    setup_s = time.perf_counter() - t_setup
+
+    # CPU baseline on a bounded sample (the first steps of the same run), and
+    # the GPU's state after the same steps for a full-size parity figure.
+    cpu = None
+    parity = None
+    if args.cpu_steps > 0:
+        from parity import compare
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        g.advance(args.cpu_steps, integrator=2)
+        gc = g.get_cells()
+        c_ref, e_ref, dts_ref, cpu_s = cpu_baseline(mesh, cells, args.cpu_steps, threads)
+        err, same = compare(gc, c_ref)
+        parity = {"steps": args.cpu_steps, "max_scaled_err": err, "bit_identical": same}
+        cpu = {"value": N * args.cpu_steps / cpu_s / 1e6, "unit": "Mcell-updates/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} Heun steps of the same {args.nx}x{args.ny} KH "
+                         f"workload, oracle/ C restatement, OpenMP {threads} threads, "
+                         f"{cpu_s:.2f} s"}
+        g.set_cells(cells)
+        g.init_extensives()
+
+    g.advance(args.warmup, integrator=2, record_dt=False)
+    if not args.no_profile:
+        g.profile(True)
+    t0 = time.perf_counter()
+    dts = g.advance(args.steps, integrator=2, record_dt=True)
+    elapsed = time.perf_counter() - t0
+    prof = g.profile_read() if not args.no_profile else {}
+    ext = g.get_extensives()
+
+    value = N * args.steps / elapsed / 1e6
+    line = {
+        "metric": "Mcell-updates/s on 1 GPU (2D, 2M cells) FP64 vs mixed; %HBM roofline; adjoint/fwd",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D Kelvin-Helmholtz {args.nx}x{args.ny} jittered lattice, "
+                               "periodic [0,2]x[0,1], LinearGaussImproved+HLLC, Heun, static mesh",
+                   "cells": N, "faces": F, "ghosts": G, "integrator": "TimeAdvance2Heun",
+                   "cfl": 0.3, "gamma": 5.0 / 3.0},
+    }
+    if prof:
+        kname, (launches, total_ms) = max(prof.items(), key=lambda kv: kv[1][1])
+        avg_s = total_ms / launches / 1e3
+        alg = kernel_alg_bytes(kname, N, F, G, nnz)
+        achieved = alg / avg_s / 1e9
+        line["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6}
+        line["kernels"] = {k: {"launches": n, "avg_us": 1e3 * ms / n,
+                               "alg_GBs": kernel_alg_bytes(k, N, F, G, nnz) / (ms / n / 1e3) / 1e9}
+                           for k, (n, ms) in sorted(prof.items())}
+    b_alg = 180 * N + 48 * F
+    stage_gbs = b_alg * 2 * args.steps / elapsed / 1e9
+    line["stage_roofline"] = {"bytes_per_stage": b_alg, "achieved": stage_gbs,
+                              "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}
+    if cpu:
+        line["cpu_baseline"] = cpu
+        line["speedup_vs_cpu"] = value / cpu["value"]
+    if parity:
+        line["parity_vs_oracle"] = parity
+    line["conservation"] = {
+        "mass": float(ext[0].sum() / ext0[0].sum() - 1.0),
+        "energy": float(ext[1].sum() / ext0[1].sum() - 1.0),
+        "momentum_x": float(ext[2].sum() - ext0[2].sum()),
+        "momentum_y": float(ext[3].sum() - ext0[3].sum()),
+    }
+    line["sim_time"] = float(np.sum(dts))
+    line["setup_s"] = setup_s
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
