@@ -50,6 +50,27 @@ def kernel_alg_bytes(name, N, F, G, nnz, moving=False):
     return 0
 
 
+def calibrate_copy(mib):
+    """One hipMemcpy device-to-device of `mib` MiB (plus one warm-up copy of
+    1 MiB), through the HIP runtime: its copy kernel reads and writes exactly
+    mib MiB, which calibrates FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(1 << 20), 3)
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # 3 = hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": n}))
+
+
 def cpu_baseline(mesh, cells, steps, threads):
     import pyoracle as po
     o = po.Oracle(mesh, order=2, nthreads=threads)
@@ -72,7 +93,13 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel HIP events")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (HIP runtime copy "
+                         "kernel) and exit: the reference point for HBM byte counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}, "

@@ -1,19 +1,27 @@
 // HIP kernels of the FP64 finite-volume step (gfx950).
 //
-// Work decomposition:
-//   k_slopes      one thread per cell: LinearGaussImproved's slope with the
-//                 shock detector and both limiters (LinearGaussImproved.cpp:67-494)
-//   k_flux2       one thread per face: face states from the slopes, rotated
-//                 HLLC, rotation back (modular_flux_calculator.cpp:39-71)
-//   k_flux1       one thread per face: first-order SimpleFluxCalculator
-//   k_update      one thread per cell: ascending-face segmented sum of
-//                 -/+ dt*A*F (SimpleExtensiveUpdater), ZeroForce, the Heun
-//                 average, IdealGas primitive recovery (SimpleCellUpdater)
-//                 and the cell's CFL term, min-reduced by wavefront
-//                 shuffles into one 64-bit atomic per wave.
-// Each cell gathers its faces through a CSR in ascending face order, which
-// is the order in which the reference scatters face contributions, so sums
-// are deterministic and equal to the reference's without atomics.
+// Per mesh (static geometry, computed once by k_geo_*):
+//   per face  FaceRec   {h = r_ij/2 seen from the first side, centroid}
+//             FaceFrame {centroid, rotated-frame normal n, |n|, parallel p, |p|}
+//             area
+//   per cell  CellGeo   {CM, width, kind, inverse gradient matrix} -- the
+//             matrix of calc_naive_slope (or the least-squares sums of
+//             GetBoundarySlope) depends on geometry only
+// Per stage:
+//   k_slopes  one thread per cell: the value-dependent half of
+//             LinearGaussImproved (gradient right-hand side in face order,
+//             shock detector, both limiters); writes a 128-byte record
+//             {cell, x-slope, y-slope, CM} read by the face kernel
+//   k_flux2   one thread per face: face states, rotated HLLC, rotate back
+//   k_flux1   one thread per face: first-order SimpleFluxCalculator
+//   k_update  one thread per cell: ascending-face segmented sum of
+//             -/+ dt*A*F, ZeroForce, Heun average, primitive recovery and
+//             the CFL term; block minima are combined by the last block
+// Precomputing a quantity once evaluates the same IEEE operations the
+// reference evaluates on every call, so results stay bit-identical.
+// Each cell gathers its faces through a CSR in ascending face order, the
+// order in which the reference scatters face contributions: sums are
+// deterministic and equal to the reference's, with no atomics.
 #pragma once
 
 #include "physics.hpp"
@@ -28,31 +36,59 @@ constexpr unsigned INFO_FACE = 0x3FFFFFFFu;
 constexpr double PI = 3.14159265358979323846;  // M_PI
 constexpr unsigned long long DBL_INF_BITS = 0x7FF0000000000000ull;
 
+enum : int { KIND_NAIVE = 0, KIND_BOUNDARY = 1, KIND_SINGULAR = 2, KIND_TOO_MANY = 3 };
+
+struct alignas(16) D2 {
+  double x, y;
+};
+struct alignas(32) D4 {  // cell {d,p,vx,vy}; extensive / flux {m,E,Mx,My}
+  double a, b, c, d;
+};
+struct alignas(64) CellGeo {
+  double cmx, cmy, width, kind;
+  double c0, c1, c2, c3;  // naive: inverse matrix i0..i3; boundary: SxSy, Sx2, Sy2, den
+};
+struct alignas(32) FaceRec {
+  double hx, hy, cx, cy;
+};
+struct alignas(64) FaceFrame {
+  double cx, cy, nx, ny, an, px, py, ap;
+};
+struct alignas(8) Inc {
+  int nbr;
+  unsigned info;
+};
+struct alignas(128) Rec {
+  double d, p, vx, vy;
+  double xd, xp, xvx, xvy;
+  double yd, yp, yvx, yvy;
+  double cmx, cmy, pad0, pad1;
+};
+
+// Raw mesh arrays (SoA, as uploaded) and the static per-mesh tables.
 struct DMesh {
   int N, F, G, bc;
   const int* ff;
   const int* fs;
   const double *v1x, *v1y, *v2x, *v2y;
-  const double *px, *py, *cmx, *cmy;
-  const double *vol, *tvol, *area;
+  const double *px, *py;
+  const D2* cm;  // [N+G]
+  const double* tvol;
+  const double* vol;
   const int* off;
-  const unsigned* info;
-  const int* nbr;
+  const Inc* inc;
   const int* gorig;  // GetOriginalIndex of each ghost
   const int* gsrc;   // real cell a ghost copies (its generator's choice)
-  const double *gnx, *gny;
+  const D2* gn;      // rigid-wall reflection normal of each ghost
   const double *wx, *wy;  // face velocities; null = zero
+  // static tables
+  const CellGeo* cgeo;
+  const FaceRec* frec;
+  const FaceFrame* fframe;
+  const double* area;
+  const double* width;  // GetWidth = sqrt(GetVolume/pi)
 };
 
-struct DCells {
-  double *d, *p, *vx, *vy;
-};
-struct DExt {
-  double *m, *E, *Mx, *My;
-};
-struct DSlopes {
-  double* s[8];  // x: d p vx vy, y: d p vx vy
-};
 struct DParams {
   double gamma, cfl, delta_v, theta, delta_p;
   int slf;
@@ -71,8 +107,9 @@ __device__ __forceinline__ void report(const DErr& e, int index, int code) {
   atomicMin(e.word, key);
 }
 
-__device__ __forceinline__ Cell load_cell(const DCells& c, int i) {
-  return Cell{c.d[i], c.p[i], c.vx[i], c.vy[i]};
+__device__ __forceinline__ Cell ld_cell(const D4* q, int i) {
+  const D4 v = q[i];
+  return Cell{v.a, v.b, v.c, v.d};
 }
 
 // Cell value at any point id: real cells directly, ghosts through the ghost
@@ -80,15 +117,15 @@ __device__ __forceinline__ Cell load_cell(const DCells& c, int i) {
 // first boundary face with the normal velocity reversed,
 // RigidWallGenerator.cpp:5-37) or PeriodicGhostGenerator (copy of the
 // original, PeriodicGhostGenerator.cpp:3-17).
-__device__ __forceinline__ Cell fetch(const DMesh& m, const DCells& c, int id) {
-  if (id < m.N) return load_cell(c, id);
+__device__ __forceinline__ Cell fetch(const DMesh& m, const D4* q, int id) {
+  if (id < m.N) return ld_cell(q, id);
   const int g = id - m.N;
-  Cell r = load_cell(c, m.gsrc[g]);
+  Cell r = ld_cell(q, m.gsrc[g]);
   if (m.bc == BC_RIGID) {
-    const double nx = m.gnx[g], ny = m.gny[g];
-    const double s2 = 2 * (r.vx * nx + r.vy * ny);
-    r.vx = r.vx - nx * s2;
-    r.vy = r.vy - ny * s2;
+    const D2 n = m.gn[g];
+    const double s2 = 2 * (r.vx * n.x + r.vy * n.y);
+    r.vx = r.vx - n.x * s2;
+    r.vy = r.vy - n.y * s2;
   }
   return r;
 }
@@ -97,6 +134,149 @@ __device__ __forceinline__ Cell sub(const Cell& a, const Cell& b) {
   return Cell{a.d - b.d, a.p - b.p, a.vx - b.vx, a.vy - b.vy};
 }
 
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Block minimum into partials[blockIdx.x]; k_reduce_min combines them in a
+// separate launch (kernel boundaries order the memory traffic; no fences).
+// Every thread of the block must call it.  Blocks of at most 1024 threads.
+__device__ __forceinline__ void block_min(double v, double* partials) {
+  __shared__ double wmin[16];
+  v = wave_min(v);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmin[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mb = wmin[0];
+    for (int k = 1; k < nw; ++k) mb = smin(mb, wmin[k]);
+    partials[blockIdx.x] = mb;
+  }
+}
+
+// One block: min of n partials into *slot (bits of a non-negative double).
+__global__ void __launch_bounds__(1024) k_reduce_min(const double* __restrict__ partials, int n,
+                                                     unsigned long long* slot) {
+  __shared__ double wmin[16];
+  double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v = smin(v, partials[k]);
+  v = wave_min(v);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) wmin[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmin[0];
+    for (int k = 1; k < nw; ++k) m = smin(m, wmin[k]);
+    *slot = static_cast<unsigned long long>(__double_as_longlong(m));
+  }
+}
+
+// ------------------------------------------------------------------ mesh --
+// Per-face static tables.
+__global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceFrame* fframe,
+                                                   double* area) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  const int a = m.ff[f], b = m.fs[f];
+  const double v1x = m.v1x[f], v1y = m.v1y[f], v2x = m.v2x[f], v2y = m.v2y[f];
+  // CalcCentroid (spatial_reconstruction.cpp:3-6)
+  const double cx = (v1x + v2x) * 0.5, cy = (v1y + v2y) * 0.5;
+  // calc_naive_slope's r_ij = normalize(r_j - r_i)*|e| for i = first
+  // (LinearGaussImproved.cpp:85-86); the second side sees -r_ij exactly
+  const double L = vabs(v2x - v1x, v2y - v1y);
+  const double dx = m.px[b] - m.px[a], dy = m.py[b] - m.py[a];
+  const double ad = vabs(dx, dy);
+  const double rx = dx / ad * L, ry = dy / ad * L;
+  frec[f] = FaceRec{rx * 0.5, ry * 0.5, cx, cy};
+  // calc_parallel_normal (modular_flux_calculator.cpp:12-22) with
+  // remove_parallel_component (simple_flux_calculator.cpp:27-31)
+  const double ppx = v2x - v1x, ppy = v2y - v1y;
+  double qx, qy;
+  if (a >= 0 && a < m.N) {
+    qx = v1x - m.px[a];
+    qy = v1y - m.py[a];
+  } else {
+    qx = m.px[b] - v1x;
+    qy = m.py[b] - v1y;
+  }
+  const double s = qx * ppx + qy * ppy;
+  const double pp = ppx * ppx + ppy * ppy;
+  const double nx = qx - ppx * s / pp, ny = qy - ppy * s / pp;
+  fframe[f] = FaceFrame{cx, cy, nx, ny, vabs(nx, ny), ppx, ppy, vabs(ppx, ppy)};
+  // SlabSymmetry::calcArea (physical_geometry.cpp:8-11)
+  area[f] = vabs(v1x - v2x, v1y - v2y);
+}
+
+// Per-cell static table: the geometric half of calc_naive_slope
+// (LinearGaussImproved.cpp:67-134: matrix, singularity test, inverse) or of
+// GetBoundarySlope (:401-422), and GetWidth (VoronoiMesh.cpp:410-413).
+__global__ void __launch_bounds__(256) k_geo_cells(DMesh m, CellGeo* cg, double* width) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const int b = m.off[i], e = m.off[i + 1], n = e - b;
+  const D2 cmi = m.cm[i];
+  CellGeo g{};
+  g.cmx = cmi.x;
+  g.cmy = cmi.y;
+  g.width = sqrt(m.tvol[i] / PI);
+  bool boundary = false;
+  for (int k = b; k < e; ++k)
+    if (m.inc[k].info & INFO_MIRROR) boundary = true;
+  if (boundary) {
+    double SxSy = 0, Sx2 = 0, Sy2 = 0;
+    for (int k = b; k < e; ++k) {
+      const D2 cj = m.cm[m.inc[k].nbr];
+      const double ey = cj.y - cmi.y, ex = cj.x - cmi.x;
+      SxSy += ey * ex;
+      Sx2 += ex * ex;
+      Sy2 += ey * ey;
+    }
+    g.kind = KIND_BOUNDARY;
+    g.c0 = SxSy;
+    g.c1 = Sx2;
+    g.c2 = Sy2;
+    g.c3 = SxSy * SxSy - Sx2 * Sy2;
+  } else if (n > 20) {
+    g.kind = KIND_TOO_MANY;
+  } else {
+    const double ptx = m.px[i], pty = m.py[i], vol = m.tvol[i];
+    double M0 = 0, M1 = 0, M2 = 0, M3 = 0;
+    for (int k = b; k < e; ++k) {
+      const int f = static_cast<int>(m.inc[k].info & INFO_FACE);
+      const int j = m.inc[k].nbr;
+      const double fx = (m.v1x[f] + m.v2x[f]) * 0.5, fy = (m.v1y[f] + m.v2y[f]) * 0.5;
+      const D2 cj = m.cm[j];
+      const double cijx = fx - (cj.x + cmi.x) * 0.5;
+      const double cijy = fy - (cj.y + cmi.y) * 0.5;
+      const double L = vabs(m.v2x[f] - m.v1x[f], m.v2y[f] - m.v1y[f]);
+      const double dx = m.px[j] - ptx, dy = m.py[j] - pty;
+      const double ad = vabs(dx, dy);
+      const double rx = dx / ad * L, ry = dy / ad * L;
+      M0 -= cijx * rx;
+      M1 -= cijy * rx;
+      M2 -= cijx * ry;
+      M3 -= cijy * ry;
+    }
+    M0 += vol;
+    M3 += vol;
+    const double det = M0 * M3 - M1 * M2;
+    if (fabs(det) < 1e-10 * vol * vol) {
+      g.kind = KIND_SINGULAR;
+    } else {
+      const double det_inv = 1.0 / det;
+      g.kind = KIND_NAIVE;
+      g.c0 = M3 * det_inv;
+      g.c1 = -M1 * det_inv;
+      g.c2 = -M2 * det_inv;
+      g.c3 = M0 * det_inv;
+    }
+  }
+  cg[i] = g;
+  width[i] = g.width;
+}
+
+// ---------------------------------------------------------------- slopes --
 // one field of slope_limit (LinearGaussImproved.cpp:232-267)
 __device__ __forceinline__ void limit_field(double dphi, double cval, double cell, double maxd,
                                             double mind, bool absguard, double& psi) {
@@ -130,33 +310,37 @@ __device__ __forceinline__ void minmax(Cell& mx, Cell& mn, const Cell& t) {
   mn.vy = smin(mn.vy, t.vy);
 }
 
-__device__ __forceinline__ void face_centroid(const DMesh& m, int f, double& cx, double& cy) {
-  cx = (m.v1x[f] + m.v2x[f]) * 0.5;
-  cy = (m.v1y[f] + m.v2y[f]) * 0.5;
-}
+// Incidences are processed in chunks of CH: the chunk's loads are all
+// issued before any of them is used (memory-level parallelism for the
+// dependent index -> data gathers); accumulation stays in face order.
+constexpr int CH = 4;
 
-// LinearGaussImproved calc_slope (LinearGaussImproved.cpp:425-494) for one cell.
-__global__ void __launch_bounds__(256) k_slopes(DMesh m, DCells c, DSlopes out, DParams prm,
-                                                DErr err) {
+// calc_slope (LinearGaussImproved.cpp:425-494) with the geometric half
+// taken from CellGeo / FaceRec.  Writes the cell's record.
+__global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
+                                                Rec* __restrict__ out, DParams prm, DErr err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
-  const int b = m.off[i], e = m.off[i + 1], n = e - b;
-  const Cell cell = load_cell(c, i);
-  bool boundary = false;
-  for (int k = b; k < e; ++k)
-    if (m.info[k] & INFO_MIRROR) boundary = true;
+  const int b = m.off[i], e = m.off[i + 1];
+  const Cell cell = ld_cell(q, i);
+  const CellGeo g = m.cgeo[i];
+  const int kind = static_cast<int>(g.kind);
   Cell sx{0, 0, 0, 0}, sy{0, 0, 0, 0};
-  const double cmx = m.cmx[i], cmy = m.cmy[i];
   double pr = 1;
   Cell amax = cell, amin = cell, bmax = cell, bmin = cell;
-  if (boundary) {
+  if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
+    report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
+    out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0, g.cmx, g.cmy, 0, 0};
+    return;
+  }
+  if (kind == KIND_BOUNDARY) {
     // GetBoundarySlope (LinearGaussImproved.cpp:401-422)
     Cell phy{0, 0, 0, 0}, phx{0, 0, 0, 0};
-    double SxSy = 0, Sx2 = 0, Sy2 = 0;
     for (int k = b; k < e; ++k) {
-      const int j = m.nbr[k];
-      const Cell nc = fetch(m, c, j);
-      const double ey = m.cmy[j] - cmy, ex = m.cmx[j] - cmx;
+      const Inc ic = m.inc[k];
+      const Cell nc = fetch(m, q, ic.nbr);
+      const D2 cj = m.cm[ic.nbr];
+      const double ey = cj.y - g.cmy, ex = cj.x - g.cmx;
       const Cell dd = sub(nc, cell);
       phy.d += dd.d * ey;
       phy.p += dd.p * ey;
@@ -166,72 +350,55 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, DCells c, DSlopes out, 
       phx.p += dd.p * ex;
       phx.vx += dd.vx * ex;
       phx.vy += dd.vy * ex;
-      SxSy += ey * ex;
-      Sx2 += ex * ex;
-      Sy2 += ey * ey;
       pr = (cell.p > nc.p) ? smin(pr, nc.p / cell.p) : smin(pr, cell.p / nc.p);
       minmax(amax, amin, nc);
-      if (!(m.info[k] & INFO_MIRROR)) minmax(bmax, bmin, nc);
+      if (!(ic.info & INFO_MIRROR)) minmax(bmax, bmin, nc);
     }
-    const double den = SxSy * SxSy - Sx2 * Sy2;
-    sx.d = (phy.d * SxSy - phx.d * Sy2) / den;
-    sx.p = (phy.p * SxSy - phx.p * Sy2) / den;
-    sx.vx = (phy.vx * SxSy - phx.vx * Sy2) / den;
-    sx.vy = (phy.vy * SxSy - phx.vy * Sy2) / den;
-    sy.d = (phx.d * SxSy - phy.d * Sx2) / den;
-    sy.p = (phx.p * SxSy - phy.p * Sx2) / den;
-    sy.vx = (phx.vx * SxSy - phy.vx * Sx2) / den;
-    sy.vy = (phx.vy * SxSy - phy.vy * Sx2) / den;
+    const double SxSy = g.c0, Sx2 = g.c1, Sy2 = g.c2, den = g.c3;
+    sx = Cell{(phy.d * SxSy - phx.d * Sy2) / den, (phy.p * SxSy - phx.p * Sy2) / den,
+              (phy.vx * SxSy - phx.vx * Sy2) / den, (phy.vy * SxSy - phx.vy * Sy2) / den};
+    sy = Cell{(phx.d * SxSy - phy.d * Sx2) / den, (phx.p * SxSy - phy.p * Sx2) / den,
+              (phx.vx * SxSy - phy.vx * Sx2) / den, (phx.vy * SxSy - phy.vy * Sx2) / den};
   } else {
-    // calc_naive_slope (LinearGaussImproved.cpp:67-134)
-    if (n > 20) {
-      report(err, i, ERR_TOO_MANY_NEIGHBORS);
-      for (int q = 0; q < 8; ++q) out.s[q][i] = 0;
-      return;
-    }
-    const double ptx = m.px[i], pty = m.py[i], vol = m.tvol[i];
-    double M0 = 0, M1 = 0, M2 = 0, M3 = 0;
+    // right-hand side of calc_naive_slope, in face order with the cell's
+    // own term first (LinearGaussImproved.cpp:91-104).  A cell on this path
+    // has no mirror face, so one set of neighbour extrema serves both limiters.
     Cell vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
-    for (int k = b; k < e; ++k) {
-      const int f = static_cast<int>(m.info[k] & INFO_FACE);
-      const int j = m.nbr[k];
-      const Cell nc = fetch(m, c, j);
-      double fx, fy;
-      face_centroid(m, f, fx, fy);
-      const double cijx = fx - (m.cmx[j] + cmx) * 0.5;
-      const double cijy = fy - (m.cmy[j] + cmy) * 0.5;
-      const double L = vabs(m.v2x[f] - m.v1x[f], m.v2y[f] - m.v1y[f]);
-      const double dx = m.px[j] - ptx, dy = m.py[j] - pty;
-      const double ad = vabs(dx, dy);
-      const double rx = dx / ad * L, ry = dy / ad * L;
-      M0 -= cijx * rx;
-      M1 -= cijy * rx;
-      M2 -= cijx * ry;
-      M3 -= cijy * ry;
-      const double hx = rx * 0.5, hy = ry * 0.5;
-      if (k == b) {
-        vcx = Cell{cell.d * hx, cell.p * hx, cell.vx * hx, cell.vy * hx};
-        vcy = Cell{cell.d * hy, cell.p * hy, cell.vx * hy, cell.vy * hy};
-      } else {
-        add_mult(vcx, cell, hx);
-        add_mult(vcy, cell, hy);
+    for (int k0 = b; k0 < e; k0 += CH) {
+      Inc ic[CH];
+      FaceRec fr[CH];
+      Cell nc[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < e) ic[u] = m.inc[k0 + u];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < e) {
+          fr[u] = m.frec[ic[u].info & INFO_FACE];
+          nc[u] = fetch(m, q, ic[u].nbr);
+        }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (k0 + u >= e) break;
+        const bool second = (ic[u].info & INFO_SECOND) != 0;
+        const double hx = second ? -fr[u].hx : fr[u].hx;
+        const double hy = second ? -fr[u].hy : fr[u].hy;
+        if (k0 + u == b) {
+          vcx = Cell{cell.d * hx, cell.p * hx, cell.vx * hx, cell.vy * hx};
+          vcy = Cell{cell.d * hy, cell.p * hy, cell.vx * hy, cell.vy * hy};
+        } else {
+          add_mult(vcx, cell, hx);
+          add_mult(vcy, cell, hy);
+        }
+        add_mult(vcy, nc[u], hy);
+        add_mult(vcx, nc[u], hx);
+        pr = (cell.p > nc[u].p) ? smin(pr, nc[u].p / cell.p) : smin(pr, cell.p / nc[u].p);
+        minmax(amax, amin, nc[u]);
       }
-      add_mult(vcy, nc, hy);
-      add_mult(vcx, nc, hx);
-      pr = (cell.p > nc.p) ? smin(pr, nc.p / cell.p) : smin(pr, cell.p / nc.p);
-      minmax(amax, amin, nc);
-      if (!(m.info[k] & INFO_MIRROR)) minmax(bmax, bmin, nc);
     }
-    M0 += vol;
-    M3 += vol;
-    const double det = M0 * M3 - M1 * M2;
-    if (fabs(det) < 1e-10 * vol * vol) {
-      report(err, i, ERR_SINGULAR_MATRIX);
-      for (int q = 0; q < 8; ++q) out.s[q][i] = 0;
-      return;
-    }
-    const double det_inv = 1.0 / det;
-    const double i0 = M3 * det_inv, i1 = -M1 * det_inv, i2 = -M2 * det_inv, i3 = M0 * det_inv;
+    bmax = amax;
+    bmin = amin;
+    const double i0 = g.c0, i1 = g.c1, i2 = g.c2, i3 = g.c3;
     sx = Cell{vcx.d * i0, vcx.p * i0, vcx.vx * i0, vcx.vy * i0};
     sy = Cell{vcy.d * i3, vcy.p * i3, vcy.vx * i3, vcy.vy * i3};
     add_mult(sx, vcy, i1);
@@ -240,32 +407,42 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, DCells c, DSlopes out, 
   if (prm.slf) {
     // is_shock (LinearGaussImproved.cpp:150-157)
     const double cs = dp2c(prm.gamma, cell.d, cell.p);
-    const double width = sqrt(m.tvol[i] / PI);
-    const bool shock = ((sx.vx + sy.vy) * width < (-prm.delta_v) * cs) || (pr < prm.delta_p);
+    const bool shock = ((sx.vx + sy.vy) * g.width < (-prm.delta_v) * cs) || (pr < prm.delta_p);
     double psi0 = 1, psi1 = 1, psi2 = 1, psi3 = 1;
     if (!shock) {
       // slope_limit (LinearGaussImproved.cpp:174-300)
       const Cell maxd = sub(bmax, cell), mind = sub(bmin, cell);
-      for (int k = b; k < e; ++k) {
-        const int f = static_cast<int>(m.info[k] & INFO_FACE);
-        double fx, fy;
-        face_centroid(m, f, fx, fy);
-        const Cell cv = interp(cell, sx, sy, fx, fy, cmx, cmy);
-        const Cell dphi = sub(cv, cell);
-        limit_field(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
-        limit_field(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
-        limit_field(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
-        limit_field(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
+      for (int k0 = b; k0 < e; k0 += CH) {
+        unsigned fi[CH];
+        D2 c2[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+          if (k0 + u < e) fi[u] = m.inc[k0 + u].info & INFO_FACE;
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+          if (k0 + u < e) {
+            const FaceRec fr = m.frec[fi[u]];
+            c2[u] = D2{fr.cx, fr.cy};
+          }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          if (k0 + u >= e) break;
+          const Cell cv = interp(cell, sx, sy, c2[u].x, c2[u].y, g.cmx, g.cmy);
+          const Cell dphi = sub(cv, cell);
+          limit_field(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
+          limit_field(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
+          limit_field(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
+          limit_field(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
+        }
       }
     } else {
       // shocked_slope_limit (LinearGaussImproved.cpp:302-399)
       const Cell maxd = sub(amax, cell), mind = sub(amin, cell);
       for (int k = b; k < e; ++k) {
-        const int f = static_cast<int>(m.info[k] & INFO_FACE);
-        const Cell nc = fetch(m, c, m.nbr[k]);
-        double fx, fy;
-        face_centroid(m, f, fx, fy);
-        const Cell cv = interp(cell, sx, sy, fx, fy, cmx, cmy);
+        const Inc ic = m.inc[k];
+        const FaceRec fr = m.frec[ic.info & INFO_FACE];
+        const Cell nc = fetch(m, q, ic.nbr);
+        const Cell cv = interp(cell, sx, sy, fr.cx, fr.cy, g.cmx, g.cmy);
         const Cell dphi = sub(cv, cell);
         shocked_field(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, prm.theta, psi0);
         shocked_field(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, prm.theta, psi1);
@@ -282,117 +459,76 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, DCells c, DSlopes out, 
     sx.vy *= psi3;
     sy.vy *= psi3;
   }
-  out.s[0][i] = sx.d;
-  out.s[1][i] = sx.p;
-  out.s[2][i] = sx.vx;
-  out.s[3][i] = sx.vy;
-  out.s[4][i] = sy.d;
-  out.s[5][i] = sy.p;
-  out.s[6][i] = sy.vx;
-  out.s[7][i] = sy.vy;
+  out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, sx.d,  sx.p,  sx.vx, sx.vy,
+               sy.d,   sy.p,   sy.vx,   sy.vy,   g.cmx, g.cmy, 0,     0};
 }
 
-__device__ __forceinline__ void load_slope(const DSlopes& s, int i, Cell& sx, Cell& sy) {
-  sx = Cell{s.s[0][i], s.s[1][i], s.s[2][i], s.s[3][i]};
-  sy = Cell{s.s[4][i], s.s[5][i], s.s[6][i], s.s[7][i]};
-}
-
+// ---------------------------------------------------------------- fluxes --
 // Face state on one side (LinearGaussImproved.cpp:571-590 for real cells,
-// :597-634 for ghosts: ghost value plus the ghost gradient -- zero for
-// rigid walls, the original's slope for periodic boxes).
-__device__ __forceinline__ Cell side_state(const DMesh& m, const DCells& c, const DSlopes& s,
-                                           int id, double fx, double fy) {
-  Cell v, sx, sy;
+// :597-634 for ghosts: the ghost value plus the ghost gradient -- zero for
+// rigid walls, the original's slope for periodic boxes -- at the ghost's CM).
+__device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Rec* rec, int id,
+                                           double fx, double fy) {
   if (id < m.N) {
-    v = load_cell(c, id);
-    load_slope(s, id, sx, sy);
-  } else {
-    v = fetch(m, c, id);
-    if (m.bc == BC_PERIODIC) {
-      load_slope(s, m.gorig[id - m.N], sx, sy);
-    } else {
-      sx = Cell{0, 0, 0, 0};
-      sy = Cell{0, 0, 0, 0};
-    }
+    const Rec r = rec[id];
+    const Cell v{r.d, r.p, r.vx, r.vy};
+    const Cell sx{r.xd, r.xp, r.xvx, r.xvy}, sy{r.yd, r.yp, r.yvx, r.yvy};
+    return interp(v, sx, sy, fx, fy, r.cmx, r.cmy);
   }
-  return interp(v, sx, sy, fx, fy, m.cmx[id], m.cmy[id]);
+  const D2 cg = m.cm[id];
+  if (m.bc == BC_PERIODIC) {
+    const Rec r = rec[m.gorig[id - m.N]];
+    const Cell v{r.d, r.p, r.vx, r.vy};
+    const Cell sx{r.xd, r.xp, r.xvx, r.xvy}, sy{r.yd, r.yp, r.yvx, r.yvy};
+    return interp(v, sx, sy, fx, fy, cg.x, cg.y);
+  }
+  const Cell z{0, 0, 0, 0};
+  return interp(fetch(m, q, id), z, z, fx, fy, cg.x, cg.y);
 }
 
-// ModularFluxCalculator body for one face (modular_flux_calculator.cpp:49-69).
-__device__ __forceinline__ int modular_face_flux(const DMesh& m, int f, const Cell& lf,
-                                                 const Cell& rt, double gamma, Cons& out) {
-  const int a = m.ff[f], b = m.fs[f];
-  const double v1x = m.v1x[f], v1y = m.v1y[f];
-  // calc_parallel_normal (modular_flux_calculator.cpp:12-22) with
-  // remove_parallel_component (simple_flux_calculator.cpp:27-31)
-  const double px = m.v2x[f] - v1x, py = m.v2y[f] - v1y;
-  double qx, qy;
-  if (a >= 0 && a < m.N) {
-    qx = v1x - m.px[a];
-    qy = v1y - m.py[a];
-  } else {
-    qx = m.px[b] - v1x;
-    qy = m.py[b] - v1y;
-  }
-  const double s = qx * px + qy * py;
-  const double pp = px * px + py * py;
-  const double nx = qx - px * s / pp, ny = qy - py * s / pp;
-  const double an = vabs(nx, ny), ap = vabs(px, py);
-  const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
-  const double speed = (nx * wx + ny * wy) / an;
-  const Prim pl = rotate(to_prim(lf, gamma), nx, ny, an, px, py, ap);
-  const Prim pr = rotate(to_prim(rt, gamma), nx, ny, an, px, py, ap);
-  Cons h;
-  const int rc = hllc(pl, pr, speed, h);
-  if (rc) return rc;
-  out = rotate_back(h, nx, ny, an, px, py, ap);
-  return 0;
-}
-
-__global__ void __launch_bounds__(256) k_flux2(DMesh m, DCells c, DSlopes s, DExt flux,
+__global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
+                                               const Rec* __restrict__ rec, D4* __restrict__ flux,
                                                DParams prm, DErr err) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
-  double fx, fy;
-  face_centroid(m, f, fx, fy);
-  const Cell lf = side_state(m, c, s, m.ff[f], fx, fy);
-  const Cell rt = side_state(m, c, s, m.fs[f], fx, fy);
-  Cons o;
-  const int rc = modular_face_flux(m, f, lf, rt, prm.gamma, o);
+  const FaceFrame fr = m.fframe[f];
+  const Cell lf = side_state(m, q, rec, m.ff[f], fr.cx, fr.cy);
+  const Cell rt = side_state(m, q, rec, m.fs[f], fr.cx, fr.cy);
+  // modular_flux_calculator.cpp:56-67
+  const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
+  const double speed = (fr.nx * wx + fr.ny * wy) / fr.an;
+  const Prim pl = rotate(to_prim(lf, prm.gamma), fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
+  const Prim pr = rotate(to_prim(rt, prm.gamma), fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
+  Cons h;
+  const int rc = hllc(pl, pr, speed, h);
+  D4 o{0, 0, 0, 0};
   if (rc) {
     report(err, f, rc);
-    o = Cons{0, 0, 0, 0};
+  } else {
+    const Cons c = rotate_back(h, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
+    o = D4{c.m, c.E, c.mx, c.my};
   }
-  flux.m[f] = o.m;
-  flux.E[f] = o.E;
-  flux.Mx[f] = o.mx;
-  flux.My[f] = o.my;
+  flux[f] = o;
 }
 
-// SpatialReconstruction output: res[f].first / .second
-__global__ void __launch_bounds__(256) k_face_states(DMesh m, DCells c, DSlopes s, DCells left,
-                                                     DCells right) {
+// SpatialReconstruction output res[f].first / .second
+__global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restrict__ q,
+                                                     const Rec* __restrict__ rec, D4* left,
+                                                     D4* right) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
-  double fx, fy;
-  face_centroid(m, f, fx, fy);
-  const Cell lf = side_state(m, c, s, m.ff[f], fx, fy);
-  const Cell rt = side_state(m, c, s, m.fs[f], fx, fy);
-  left.d[f] = lf.d;
-  left.p[f] = lf.p;
-  left.vx[f] = lf.vx;
-  left.vy[f] = lf.vy;
-  right.d[f] = rt.d;
-  right.p[f] = rt.p;
-  right.vx[f] = rt.vx;
-  right.vy[f] = rt.vy;
+  const FaceFrame fr = m.fframe[f];
+  const Cell lf = side_state(m, q, rec, m.ff[f], fr.cx, fr.cy);
+  const Cell rt = side_state(m, q, rec, m.fs[f], fr.cx, fr.cy);
+  left[f] = D4{lf.d, lf.p, lf.vx, lf.vy};
+  right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
 }
 
 // SimpleFluxCalculator body (simple_flux_calculator.cpp:80-126,167-189):
 // normal = point difference; a mirror face reflects the real state about the
 // face (Reflect, geometry.cpp:117-120).
-__global__ void __launch_bounds__(256) k_flux1(DMesh m, DCells c, DExt flux, DParams prm,
-                                               DErr err) {
+__global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q,
+                                               D4* __restrict__ flux, DParams prm, DErr err) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const int a = m.ff[f], b = m.fs[f];
@@ -402,74 +538,66 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, DCells c, DExt flux, DPa
   const int oa = fa ? a : m.gorig[a - m.N];
   const int ob = fb ? b : m.gorig[b - m.N];
   const bool mirror = oa == ob;  // GetOriginalIndex equal on both sides
-  Prim l, r;
   auto reflect = [&](const Prim& s) {
-    Prim q = s;
+    Prim r = s;
     const double s2 = 2 * (s.vx * px + s.vy * py);
     const double ap = vabs(px, py);
     const double a2 = ap * ap;
-    q.vx = px * s2 / a2 - s.vx;
-    q.vy = py * s2 / a2 - s.vy;
-    return q;
+    r.vx = px * s2 / a2 - s.vx;
+    r.vy = py * s2 / a2 - s.vy;
+    return r;
   };
+  Prim l, r;
   if (!fa) {
-    r = to_prim(load_cell(c, b), prm.gamma);
-    l = mirror ? reflect(r) : to_prim(fetch(m, c, a), prm.gamma);
+    r = to_prim(ld_cell(q, b), prm.gamma);
+    l = mirror ? reflect(r) : to_prim(fetch(m, q, a), prm.gamma);
   } else if (!fb) {
-    l = to_prim(load_cell(c, a), prm.gamma);
-    r = mirror ? reflect(l) : to_prim(fetch(m, c, b), prm.gamma);
+    l = to_prim(ld_cell(q, a), prm.gamma);
+    r = mirror ? reflect(l) : to_prim(fetch(m, q, b), prm.gamma);
   } else {
-    l = to_prim(load_cell(c, a), prm.gamma);
-    r = to_prim(load_cell(c, b), prm.gamma);
+    l = to_prim(ld_cell(q, a), prm.gamma);
+    r = to_prim(ld_cell(q, b), prm.gamma);
   }
   const double an = vabs(nx, ny), ap = vabs(px, py);
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
   const double velocity = (nx * wx + ny * wy) / an;
-  Cons h, o;
+  Cons h;
   const int rc = hllc(rotate(l, nx, ny, an, px, py, ap), rotate(r, nx, ny, an, px, py, ap),
                       velocity, h);
+  D4 o{0, 0, 0, 0};
   if (rc) {
     report(err, f, rc);
-    o = Cons{0, 0, 0, 0};
   } else {
-    o = rotate_back(h, nx, ny, an, px, py, ap);
+    const Cons c = rotate_back(h, nx, ny, an, px, py, ap);
+    o = D4{c.m, c.E, c.mx, c.my};
   }
-  flux.m[f] = o.m;
-  flux.E[f] = o.E;
-  flux.Mx[f] = o.mx;
-  flux.My[f] = o.my;
+  flux[f] = o;
 }
 
+// ------------------------------------------------------------------- CFL --
 // TimeStepCalculator::operator[] (simple_cfl.cpp:28-47) for one cell.
-__device__ __forceinline__ double cfl_inv(const DMesh& m, int i, double gamma, const Cell& q) {
-  const double radius = sqrt(m.tvol[i] / PI);
-  const double cs = dp2c(gamma, q.d, q.p);
+__device__ __forceinline__ double cfl_inv(const DMesh& m, int i, double gamma, double width,
+                                          const Cell& c) {
+  const double cs = dp2c(gamma, c.d, c.p);
   double res = 0;
   if (m.wx) {
     for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
-      const int f = static_cast<int>(m.info[k] & INFO_FACE);
-      res = fmax(res, (cs + vabs(q.vx - m.wx[f], q.vy - m.wy[f])) / radius);
+      const int f = static_cast<int>(m.inc[k].info & INFO_FACE);
+      res = fmax(res, (cs + vabs(c.vx - m.wx[f], c.vy - m.wy[f])) / width);
     }
   } else if (m.off[i + 1] > m.off[i]) {
     // all face velocities zero: every face gives the same term
-    res = fmax(res, (cs + vabs(q.vx - 0.0, q.vy - 0.0)) / radius);
+    res = fmax(res, (cs + vabs(c.vx - 0.0, c.vy - 0.0)) / width);
   }
   return 1.0 / res;
 }
 
-// wavefront min, then one 64-bit atomic per wave (positive doubles order
-// like their bit patterns)
-__device__ __forceinline__ void wave_min_to(unsigned long long* slot, double v) {
-  for (int o = 32; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMin(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
-}
-
-__global__ void __launch_bounds__(256) k_cfl(DMesh m, DCells c, double gamma,
-                                             unsigned long long* slot) {
+__global__ void __launch_bounds__(256) k_cfl(DMesh m, const D4* __restrict__ q, double gamma,
+                                             double* partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
-  if (i < m.N) v = cfl_inv(m, i, gamma, load_cell(c, i));
-  wave_min_to(slot, v);
+  if (i < m.N) v = cfl_inv(m, i, gamma, m.width[i], ld_cell(q, i));
+  block_min(v, partials);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
@@ -477,23 +605,26 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
   if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------- update --
 enum : int {
-  UPD_ZF_BEFORE = 1,   // ZeroForce before the face sum (TimeAdvance2Heun)
-  UPD_ZF_AFTER = 2,    // ZeroForce after the face sum (TimeAdvance)
-  UPD_AVERAGE = 4,     // 0.5*(mid + old) (average_extensive)
-  UPD_CELLS = 8,       // SimpleCellUpdater
-  UPD_CFL = 16,        // next step's CFL term
-  UPD_FACES = 32,      // apply the face fluxes
+  UPD_ZF_BEFORE = 1,  // ZeroForce before the face sum (TimeAdvance2Heun)
+  UPD_ZF_AFTER = 2,   // ZeroForce after the face sum (TimeAdvance)
+  UPD_AVERAGE = 4,    // 0.5*(mid + old) (average_extensive)
+  UPD_CELLS = 8,      // SimpleCellUpdater
+  UPD_CFL = 16,       // next step's CFL term
+  UPD_FACES = 32,     // apply the face fluxes
 };
 
 struct UpdArgs {
-  DExt in, out, avg;
-  DExt flux;
-  DCells cells;
+  const D4* in;
+  D4* out;
+  const D4* avg;
+  const D4* flux;
+  D4* cells;
   double dt;                          // used when dt_slot is null
-  const unsigned long long* dt_slot;  // min_i 1/res_i from the CFL reduction
+  const unsigned long long* dt_slot;  // min_i 1/res_i of the CFL reduction
   unsigned long long* reset_slot;     // set to +inf (next step's slot)
-  unsigned long long* cfl_slot;       // receives this step's min
+  double* partials;                   // per-block CFL minima
   double* dt_hist;
   int step;
   int flags;
@@ -513,7 +644,8 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
   }
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   if (i < m.N) {
-    double em = a.in.m[i], eE = a.in.E[i], ex = a.in.Mx[i], ey = a.in.My[i];
+    const D4 x = a.in[i];
+    double em = x.a, eE = x.b, ex = x.c, ey = x.d;
     if (a.flags & UPD_ZF_BEFORE) {
       em += dt * 0.0;
       ex += 0.0 * dt;
@@ -521,22 +653,37 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
       eE += dt * 0.0;
     }
     if (a.flags & UPD_FACES) {
-      for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
-        const unsigned inf = m.info[k];
-        const int f = static_cast<int>(inf & INFO_FACE);
-        const double s = dt * m.area[f];
-        const double dm = a.flux.m[f] * s, dE = a.flux.E[f] * s, dx = a.flux.Mx[f] * s,
-                     dy = a.flux.My[f] * s;
-        if (inf & INFO_SECOND) {
-          em += dm;
-          eE += dE;
-          ex += dx;
-          ey += dy;
-        } else {
-          em -= dm;
-          eE -= dE;
-          ex -= dx;
-          ey -= dy;
+      const int kb = m.off[i], ke = m.off[i + 1];
+      for (int k0 = kb; k0 < ke; k0 += CH) {
+        unsigned inf[CH];
+        D4 fl[CH];
+        double ar[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+          if (k0 + u < ke) inf[u] = m.inc[k0 + u].info;
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+          if (k0 + u < ke) {
+            const int f = static_cast<int>(inf[u] & INFO_FACE);
+            fl[u] = a.flux[f];
+            ar[u] = m.area[f];
+          }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          if (k0 + u >= ke) break;
+          const double s = dt * ar[u];
+          const double dm = fl[u].a * s, dE = fl[u].b * s, dx = fl[u].c * s, dy = fl[u].d * s;
+          if (inf[u] & INFO_SECOND) {
+            em += dm;
+            eE += dE;
+            ex += dx;
+            ey += dy;
+          } else {
+            em -= dm;
+            eE -= dE;
+            ex -= dx;
+            ey -= dy;
+          }
         }
       }
     }
@@ -547,43 +694,55 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
       eE += dt * 0.0;
     }
     if (a.flags & UPD_AVERAGE) {
-      em = 0.5 * (em + a.avg.m[i]);
-      eE = 0.5 * (eE + a.avg.E[i]);
-      ex = (ex + a.avg.Mx[i]) * 0.5;
-      ey = (ey + a.avg.My[i]) * 0.5;
+      const D4 o = a.avg[i];
+      em = 0.5 * (em + o.a);
+      eE = 0.5 * (eE + o.b);
+      ex = (ex + o.c) * 0.5;
+      ey = (ey + o.d) * 0.5;
     }
-    a.out.m[i] = em;
-    a.out.E[i] = eE;
-    a.out.Mx[i] = ex;
-    a.out.My[i] = ey;
+    a.out[i] = D4{em, eE, ex, ey};
     if (a.flags & UPD_CELLS) {
       const double d = em / m.vol[i];
       const double vx = ex / em, vy = ey / em;
       const double e = eE / em - 0.5 * (vx * vx + vy * vy);
-      if (d < 0) report(err, i, ERR_NEGATIVE_DENSITY);
-      else if (e < 0) report(err, i, ERR_NEGATIVE_THERMAL);
+      if (d < 0)
+        report(err, i, ERR_NEGATIVE_DENSITY);
+      else if (e < 0)
+        report(err, i, ERR_NEGATIVE_THERMAL);
       const double p = (prm.gamma - 1) * e * d;
-      a.cells.d[i] = d;
-      a.cells.p[i] = p;
-      a.cells.vx[i] = vx;
-      a.cells.vy[i] = vy;
-      if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, Cell{d, p, vx, vy});
+      a.cells[i] = D4{d, p, vx, vy};
+      if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, m.width[i], Cell{d, p, vx, vy});
     }
   }
-  if (a.flags & UPD_CFL) wave_min_to(a.cfl_slot, cfl_v);
+  if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
 }
 
 // init_extensives (hdsim2d.cpp:41-66)
-__global__ void __launch_bounds__(256) k_init_ext(DMesh m, DCells c, DExt x, double gamma) {
+__global__ void __launch_bounds__(256) k_init_ext(DMesh m, const D4* __restrict__ q, D4* x,
+                                                  double gamma) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const double volume = m.vol[i];
-  const double d = c.d[i], p = c.p[i], vx = c.vx[i], vy = c.vy[i];
-  const double mass = volume * d;
-  x.m[i] = mass;
-  x.E[i] = dp2e(gamma, d, p) * mass + 0.5 * mass * (vx * vx + vy * vy);
-  x.Mx[i] = vx * mass;
-  x.My[i] = vy * mass;
+  const D4 c = q[i];
+  const double mass = volume * c.a;
+  x[i] = D4{mass, dp2e(gamma, c.a, c.b) * mass + 0.5 * mass * (c.c * c.c + c.d * c.d), c.c * mass,
+            c.d * mass};
+}
+
+// SoA <-> AoS for the host API
+__global__ void k_pack4(const double* __restrict__ s, D4* __restrict__ d, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = D4{s[i], s[n + i], s[2 * static_cast<size_t>(n) + i], s[3 * static_cast<size_t>(n) + i]};
+}
+__global__ void k_unpack4(const D4* __restrict__ s, double* __restrict__ d, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const D4 v = s[i];
+    d[i] = v.a;
+    d[n + i] = v.b;
+    d[2 * static_cast<size_t>(n) + i] = v.c;
+    d[3 * static_cast<size_t>(n) + i] = v.d;
+  }
 }
 
 }  // namespace rg

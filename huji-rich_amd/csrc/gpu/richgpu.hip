@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,11 +27,15 @@ enum KernelId : int {
   KID_FACE_STATES,
   KID_INIT_EXT,
   KID_FILL,
+  KID_GEO,
+  KID_PACK,
+  KID_REDUCE,
   KID_COUNT
 };
-const char* const kKernelNames[KID_COUNT] = {"k_cfl",    "k_slopes",      "k_flux2",
-                                             "k_flux1",  "k_update",      "k_face_states",
-                                             "k_init_ext", "k_fill_u64"};
+const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
+                                             "k_flux1",       "k_update",   "k_face_states",
+                                             "k_init_ext",    "k_fill_u64", "k_geo",
+                                             "k_pack",        "k_reduce_min"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -58,20 +63,19 @@ struct rg_ctx {
   rg_params prm{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  int device = 0;
 
   int N = 0, F = 0, G = 0, nnz = 0;
   bool has_mesh = false, has_cells = false, has_ext = false, has_flux = false;
-  bool has_w = false;
 
-  std::vector<void*> mesh_bufs;   // freed on re-mesh
-  std::vector<void*> state_bufs;  // freed on re-mesh
+  std::vector<void*> mesh_bufs;  // freed on re-mesh
   int64_t bytes = 0;
 
   DMesh dm{};
-  DCells cells{}, mid_cells{}, left{}, right{};
-  DExt ext{}, mid_ext{}, flux{};
-  DSlopes slopes{};
+  D4 *prim = nullptr, *mid_prim = nullptr, *ext = nullptr, *mid_ext = nullptr, *flux = nullptr;
+  D4 *left = nullptr, *right = nullptr;
+  Rec* rec = nullptr;
+  double* staging = nullptr;   // 4*max(N,F) doubles for SoA host transfers
+  double* partials = nullptr;  // per-block minima of the CFL reduction
   double* wbuf = nullptr;
   unsigned long long* words = nullptr;  // [0],[1] dt slots, [2] error word
   unsigned long long* host_words = nullptr;
@@ -98,27 +102,40 @@ int fail_hip(rg_ctx* c, hipError_t e) {
   return RG_E_HIP;
 }
 
-#define HIPCHK(c, expr)                     \
-  do {                                      \
-    const hipError_t e_ = (expr);           \
-    if (e_ != hipSuccess) return fail_hip((c), e_); \
+#define HIPCHK(c, expr)                              \
+  do {                                               \
+    const hipError_t e_ = (expr);                    \
+    if (e_ != hipSuccess) return fail_hip((c), e_);  \
   } while (0)
 
 template <class T>
-hipError_t dalloc(rg_ctx* c, std::vector<void*>& owner, T** p, size_t n) {
+hipError_t dalloc(rg_ctx* c, T** p, size_t n) {
   void* q = nullptr;
   const size_t bytes = sizeof(T) * (n ? n : 1);
   const hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) return e;
-  owner.push_back(q);
+  c->mesh_bufs.push_back(q);
   c->bytes += static_cast<int64_t>(bytes);
   *p = static_cast<T*>(q);
   return hipSuccess;
 }
 
-void free_all(rg_ctx* c, std::vector<void*>& owner) {
-  for (void* p : owner) (void)hipFree(p);
-  owner.clear();
+template <class T>
+hipError_t upload(rg_ctx* c, T** p, const T* src, size_t n) {
+  hipError_t e = dalloc(c, p, n);
+  if (e == hipSuccess && n)
+    e = hipMemcpyAsync(*p, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream);
+  return e;
+}
+
+void free_mesh(rg_ctx* c) {
+  for (void* p : c->mesh_bufs) (void)hipFree(p);
+  c->mesh_bufs.clear();
+  c->bytes = 0;
+  c->prim = c->mid_prim = c->ext = c->mid_ext = c->flux = c->left = c->right = nullptr;
+  c->rec = nullptr;
+  c->staging = c->partials = c->wbuf = nullptr;
+  c->has_mesh = c->has_cells = c->has_ext = c->has_flux = false;
 }
 
 hipEvent_t ev_get(rg_ctx* c) {
@@ -171,6 +188,12 @@ DParams dparams(const rg_ctx* c) {
 }
 
 DErr derr(rg_ctx* c, int step, int phase) { return DErr{c->words + 2, step, phase}; }
+hipError_t run_reduce_min(rg_ctx* c, int slot) {
+  return launch(c, KID_REDUCE, [&] {
+    k_reduce_min<<<1, 1024, 0, c->stream>>>(c->partials, static_cast<int>(blocks_for(c->N)),
+                                            c->words + slot);
+  });
+}
 
 hipError_t fill_words(rg_ctx* c, int first, int n, unsigned long long v) {
   return launch(c, KID_FILL, [&] {
@@ -195,8 +218,7 @@ int collect_error(rg_ctx* c) {
   e.step = static_cast<int32_t>(w >> 39);
   std::snprintf(e.message, sizeof(e.message), "%s", err_message(e.code));
   c->last = e;
-  // clear for the next call
-  (void)fill_words(c, 2, 1, ~0ull);
+  (void)fill_words(c, 2, 1, ~0ull);  // clear for the next call
   return RG_E_DEVICE;
 }
 
@@ -204,20 +226,20 @@ bool ready(const rg_ctx* c, bool need_cells, bool need_ext) {
   return c && c->has_mesh && (!need_cells || c->has_cells) && (!need_ext || c->has_ext);
 }
 
-hipError_t run_slopes(rg_ctx* c, const DCells& cells, int step, int phase) {
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase) {
   return launch(c, KID_SLOPES, [&] {
-    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, c->slopes, dparams(c),
+    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, c->rec, dparams(c),
                                                          derr(c, step, phase));
   });
 }
 
 // FluxCalculator on `cells` into c->flux.
-hipError_t run_flux(rg_ctx* c, const DCells& cells, int step, int phase_base) {
+hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base) {
   if (c->prm.order == 2) {
-    hipError_t e = run_slopes(c, cells, step, phase_base);
+    const hipError_t e = run_slopes(c, cells, step, phase_base);
     if (e != hipSuccess) return e;
     return launch(c, KID_FLUX2, [&] {
-      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, c->slopes, c->flux,
+      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, c->rec, c->flux,
                                                            dparams(c), derr(c, step, phase_base + 1));
     });
   }
@@ -234,9 +256,38 @@ hipError_t run_update(rg_ctx* c, const UpdArgs& a, int step, int phase) {
   });
 }
 
-UpdArgs upd_args() {
-  UpdArgs a{};
-  return a;
+hipError_t run_cfl(rg_ctx* c, const D4* cells, int slot) {
+  const hipError_t e = launch(c, KID_CFL, [&] {
+    k_cfl<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, c->prm.gamma, c->partials);
+  });
+  if (e != hipSuccess) return e;
+  return run_reduce_min(c, slot);
+}
+
+// host SoA [4][n] <-> device AoS
+int put4(rg_ctx* c, D4* dst, const double* const src[4], int n) {
+  for (int k = 0; k < 4; ++k)
+    if (!src[k]) return RG_E_ARG;
+  for (int k = 0; k < 4; ++k)
+    HIPCHK(c, hipMemcpyAsync(c->staging + static_cast<size_t>(k) * n, src[k], sizeof(double) * n,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch(c, KID_PACK, [&] {
+    k_pack4<<<blocks_for(n), kBlock, 0, c->stream>>>(c->staging, dst, n);
+  }));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RG_OK;
+}
+
+int get4(rg_ctx* c, double* const dst[4], const D4* src, int n) {
+  HIPCHK(c, launch(c, KID_PACK, [&] {
+    k_unpack4<<<blocks_for(n), kBlock, 0, c->stream>>>(src, c->staging, n);
+  }));
+  for (int k = 0; k < 4; ++k)
+    if (dst[k])
+      HIPCHK(c, hipMemcpyAsync(dst[k], c->staging + static_cast<size_t>(k) * n, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RG_OK;
 }
 
 }  // namespace
@@ -252,17 +303,15 @@ int rg_create(const rg_params* params, rg_ctx** out) {
   rg_ctx* c = new (std::nothrow) rg_ctx();
   if (!c) return RG_E_ARG;
   c->prm = *params;
-  hipError_t e = hipGetDevice(&c->device);
-  if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+  hipError_t e = hipStreamCreate(&c->stream);
   if (e == hipSuccess) c->own_stream = true;
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->host_words),
-                                         4 * sizeof(unsigned long long), hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&c->host_words), 4 * sizeof(unsigned long long),
+                      hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->words), 4 * sizeof(unsigned long long));
+  if (e == hipSuccess) e = fill_words(c, 0, 4, ~0ull);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
-    rg_destroy(c);
-    return RG_E_HIP;
-  }
-  if (fill_words(c, 0, 4, ~0ull) != hipSuccess) {
     rg_destroy(c);
     return RG_E_HIP;
   }
@@ -273,8 +322,7 @@ int rg_create(const rg_params* params, rg_ctx** out) {
 void rg_destroy(rg_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  free_all(c, c->mesh_bufs);
-  free_all(c, c->state_bufs);
+  free_mesh(c);
   if (c->dt_hist) (void)hipFree(c->dt_hist);
   if (c->words) (void)hipFree(c->words);
   if (c->host_words) (void)hipHostFree(c->host_words);
@@ -324,8 +372,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   for (int g = 0; g < G; ++g)
     if (m->ghost_orig[g] < 0 || m->ghost_orig[g] >= N) return RG_E_ARG;
   auto orig = [&](int id) { return id < N ? id : m->ghost_orig[id - N]; };
-  std::vector<unsigned> info(static_cast<size_t>(nnz));
-  std::vector<int> nbr(static_cast<size_t>(nnz));
+  std::vector<Inc> inc(static_cast<size_t>(nnz));
   for (int i = 0; i < N; ++i) {
     int prev = -1;
     for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
@@ -337,17 +384,17 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       unsigned w = static_cast<unsigned>(f);
       if (b == i) w |= INFO_SECOND;
       if (orig(a) == orig(b)) w |= INFO_MIRROR;
-      info[k] = w;
-      nbr[k] = (a == i) ? b : a;
+      inc[k] = Inc{(a == i) ? b : a, w};
     }
   }
-  // ghost sources: RigidWallGenerator copies the real cell of the ghost's
-  // first boundary face (flat_map insert keeps the first) and reverses the
-  // velocity along the ghost-minus-real direction; PeriodicGhostGenerator
-  // copies the original.
-  std::vector<int> gsrc(static_cast<size_t>(G));
-  std::vector<double> gnx(static_cast<size_t>(G), 0.0), gny(static_cast<size_t>(G), 0.0);
-  std::vector<char> seen(static_cast<size_t>(G), 0);
+  // Ghost sources: RigidWallGenerator copies the real cell of the ghost's
+  // first boundary face (its flat_map insert keeps the first) and reverses
+  // the velocity along the ghost-minus-real direction
+  // (RigidWallGenerator.cpp:5-37); PeriodicGhostGenerator copies the
+  // original (PeriodicGhostGenerator.cpp:3-17).
+  std::vector<int> gsrc(static_cast<size_t>(std::max(G, 1)), 0);
+  std::vector<D2> gn(static_cast<size_t>(std::max(G, 1)), D2{0.0, 0.0});
+  std::vector<char> seen(static_cast<size_t>(std::max(G, 1)), 0);
   for (int g = 0; g < G; ++g) gsrc[g] = m->ghost_orig[g];
   if (c->prm.bc == RG_BC_RIGID) {
     for (int f = 0; f < F; ++f) {
@@ -357,8 +404,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       const int g = (side == 1 ? a : b) - N;
       if (seen[g]) continue;
       seen[g] = 1;
-      const int real = side == 1 ? b : a;
-      gsrc[g] = real;
+      gsrc[g] = side == 1 ? b : a;
       double nx, ny;
       if (side == 1) {
         nx = m->pt_x[a] - m->pt_x[b];
@@ -368,110 +414,87 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
         ny = m->pt_y[b] - m->pt_y[a];
       }
       const double an = std::sqrt(nx * nx + ny * ny);
-      gnx[g] = nx / an;
-      gny[g] = ny / an;
+      gn[g] = D2{nx / an, ny / an};
     }
   }
-  // SlabSymmetry::calcArea (physical_geometry.cpp:8-11)
-  std::vector<double> area(static_cast<size_t>(F));
-  for (int f = 0; f < F; ++f) {
-    const double dx = m->v1x[f] - m->v2x[f], dy = m->v1y[f] - m->v2y[f];
-    area[f] = std::sqrt(dx * dx + dy * dy);
-  }
+  std::vector<D2> cm(static_cast<size_t>(N) + G);
+  for (int i = 0; i < N + G; ++i) cm[i] = D2{m->cm_x[i], m->cm_y[i]};
   // --- upload
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  free_all(c, c->mesh_bufs);
-  free_all(c, c->state_bufs);
-  c->bytes = 0;
-  c->has_mesh = c->has_cells = c->has_ext = c->has_flux = c->has_w = false;
-  c->wbuf = nullptr;
-  c->left = DCells{};
-  c->right = DCells{};
+  free_mesh(c);
   const size_t NG = static_cast<size_t>(N) + G;
   DMesh d{};
   d.N = N;
   d.F = F;
   d.G = G;
   d.bc = c->prm.bc;
-  auto up = [&](auto** dst, const auto* src, size_t n) -> hipError_t {
-    using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
-    T* p = nullptr;
-    hipError_t e = dalloc(c, c->mesh_bufs, &p, n);
-    if (e != hipSuccess) return e;
-    if (n) e = hipMemcpyAsync(p, src, sizeof(T) * n, hipMemcpyHostToDevice, c->stream);
-    *dst = p;
-    return e;
-  };
-  int* p_ff = nullptr;
-  int* p_fs = nullptr;
-  double *p_v1x, *p_v1y, *p_v2x, *p_v2y, *p_px, *p_py, *p_cmx, *p_cmy, *p_vol, *p_tvol, *p_area;
-  int *p_off, *p_nbr, *p_gorig, *p_gsrc;
-  unsigned* p_info;
-  double *p_gnx, *p_gny;
-  const int zero_int = 0;
-  HIPCHK(c, up(&p_ff, m->face_first, F));
-  HIPCHK(c, up(&p_fs, m->face_second, F));
-  HIPCHK(c, up(&p_v1x, m->v1x, F));
-  HIPCHK(c, up(&p_v1y, m->v1y, F));
-  HIPCHK(c, up(&p_v2x, m->v2x, F));
-  HIPCHK(c, up(&p_v2y, m->v2y, F));
-  HIPCHK(c, up(&p_px, m->pt_x, NG));
-  HIPCHK(c, up(&p_py, m->pt_y, NG));
-  HIPCHK(c, up(&p_cmx, m->cm_x, NG));
-  HIPCHK(c, up(&p_cmy, m->cm_y, NG));
-  HIPCHK(c, up(&p_vol, m->volume, N));
-  HIPCHK(c, up(&p_tvol, m->tess_volume, N));
-  HIPCHK(c, up(&p_area, area.data(), F));
-  HIPCHK(c, up(&p_off, m->csr_off, static_cast<size_t>(N) + 1));
-  HIPCHK(c, up(&p_info, info.data(), static_cast<size_t>(nnz)));
-  HIPCHK(c, up(&p_nbr, nbr.data(), static_cast<size_t>(nnz)));
-  HIPCHK(c, up(&p_gorig, G ? m->ghost_orig : &zero_int, G ? G : 1));
-  HIPCHK(c, up(&p_gsrc, G ? gsrc.data() : &zero_int, G ? G : 1));
-  const double zd = 0.0;
-  HIPCHK(c, up(&p_gnx, G ? gnx.data() : &zd, G ? G : 1));
-  HIPCHK(c, up(&p_gny, G ? gny.data() : &zd, G ? G : 1));
-  d.ff = p_ff;
-  d.fs = p_fs;
-  d.v1x = p_v1x;
-  d.v1y = p_v1y;
-  d.v2x = p_v2x;
-  d.v2y = p_v2y;
-  d.px = p_px;
-  d.py = p_py;
-  d.cmx = p_cmx;
-  d.cmy = p_cmy;
-  d.vol = p_vol;
-  d.tvol = p_tvol;
-  d.area = p_area;
-  d.off = p_off;
-  d.info = p_info;
-  d.nbr = p_nbr;
-  d.gorig = p_gorig;
-  d.gsrc = p_gsrc;
-  d.gnx = p_gnx;
-  d.gny = p_gny;
-  d.wx = nullptr;
-  d.wy = nullptr;
-  // --- state buffers
-  auto alloc4 = [&](double** q, size_t n) -> hipError_t {
-    for (int k = 0; k < 4; ++k) {
-      const hipError_t e = dalloc(c, c->state_bufs, &q[k], n);
-      if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-  };
-  double* q[4];
-  HIPCHK(c, alloc4(q, N));
-  c->cells = DCells{q[0], q[1], q[2], q[3]};
-  HIPCHK(c, alloc4(q, N));
-  c->mid_cells = DCells{q[0], q[1], q[2], q[3]};
-  HIPCHK(c, alloc4(q, N));
-  c->ext = DExt{q[0], q[1], q[2], q[3]};
-  HIPCHK(c, alloc4(q, N));
-  c->mid_ext = DExt{q[0], q[1], q[2], q[3]};
-  HIPCHK(c, alloc4(q, F));
-  c->flux = DExt{q[0], q[1], q[2], q[3]};
-  for (int k = 0; k < 8; ++k) HIPCHK(c, dalloc(c, c->state_bufs, &c->slopes.s[k], N));
+  int *ff, *fs, *off, *gorig, *gsrc_d;
+  double *v1x, *v1y, *v2x, *v2y, *px, *py, *tvol, *vol, *area, *width;
+  Inc* inc_d;
+  D2 *cm_d, *gn_d;
+  CellGeo* cgeo;
+  FaceRec* frec;
+  FaceFrame* fframe;
+  const int zero = 0;
+  HIPCHK(c, upload(c, &ff, m->face_first, F));
+  HIPCHK(c, upload(c, &fs, m->face_second, F));
+  HIPCHK(c, upload(c, &v1x, m->v1x, F));
+  HIPCHK(c, upload(c, &v1y, m->v1y, F));
+  HIPCHK(c, upload(c, &v2x, m->v2x, F));
+  HIPCHK(c, upload(c, &v2y, m->v2y, F));
+  HIPCHK(c, upload(c, &px, m->pt_x, NG));
+  HIPCHK(c, upload(c, &py, m->pt_y, NG));
+  HIPCHK(c, upload(c, &tvol, m->tess_volume, N));
+  HIPCHK(c, upload(c, &vol, m->volume, N));
+  HIPCHK(c, upload(c, &off, m->csr_off, static_cast<size_t>(N) + 1));
+  HIPCHK(c, upload(c, &inc_d, inc.data(), static_cast<size_t>(nnz)));
+  HIPCHK(c, upload(c, &cm_d, cm.data(), NG));
+  HIPCHK(c, upload(c, &gorig, G ? m->ghost_orig : &zero, G ? G : 1));
+  HIPCHK(c, upload(c, &gsrc_d, gsrc.data(), gsrc.size()));
+  HIPCHK(c, upload(c, &gn_d, gn.data(), gn.size()));
+  HIPCHK(c, dalloc(c, &cgeo, N));
+  HIPCHK(c, dalloc(c, &frec, F));
+  HIPCHK(c, dalloc(c, &fframe, F));
+  HIPCHK(c, dalloc(c, &area, F));
+  HIPCHK(c, dalloc(c, &width, N));
+  d.ff = ff;
+  d.fs = fs;
+  d.v1x = v1x;
+  d.v1y = v1y;
+  d.v2x = v2x;
+  d.v2y = v2y;
+  d.px = px;
+  d.py = py;
+  d.cm = cm_d;
+  d.tvol = tvol;
+  d.vol = vol;
+  d.off = off;
+  d.inc = inc_d;
+  d.gorig = gorig;
+  d.gsrc = gsrc_d;
+  d.gn = gn_d;
+  d.wx = d.wy = nullptr;
+  d.cgeo = cgeo;
+  d.frec = frec;
+  d.fframe = fframe;
+  d.area = area;
+  d.width = width;
+  // static geometry tables
+  HIPCHK(c, launch(c, KID_GEO, [&] {
+    k_geo_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, frec, fframe, area);
+  }));
+  HIPCHK(c, launch(c, KID_GEO, [&] {
+    k_geo_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeo, width);
+  }));
+  // state
+  HIPCHK(c, dalloc(c, &c->prim, N));
+  HIPCHK(c, dalloc(c, &c->mid_prim, N));
+  HIPCHK(c, dalloc(c, &c->ext, N));
+  HIPCHK(c, dalloc(c, &c->mid_ext, N));
+  HIPCHK(c, dalloc(c, &c->flux, F));
+  HIPCHK(c, dalloc(c, &c->rec, N));
+  HIPCHK(c, dalloc(c, &c->staging, 4 * static_cast<size_t>(std::max(N, F))));
+  HIPCHK(c, dalloc(c, &c->partials, blocks_for(std::max(N, F))));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->dm = d;
   c->N = N;
@@ -486,42 +509,22 @@ int rg_set_face_velocity(rg_ctx* c, const double* wx, const double* wy) {
   if (!c || !c->has_mesh || (!wx) != (!wy)) return RG_E_ARG;
   if (!wx) {
     c->dm.wx = c->dm.wy = nullptr;
-    c->has_w = false;
     return RG_OK;
   }
-  if (!c->wbuf) HIPCHK(c, dalloc(c, c->mesh_bufs, &c->wbuf, 2 * static_cast<size_t>(c->F)));
+  if (!c->wbuf) HIPCHK(c, dalloc(c, &c->wbuf, 2 * static_cast<size_t>(c->F)));
   HIPCHK(c, hipMemcpyAsync(c->wbuf, wx, sizeof(double) * c->F, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->wbuf + c->F, wy, sizeof(double) * c->F, hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->dm.wx = c->wbuf;
   c->dm.wy = c->wbuf + c->F;
-  c->has_w = true;
-  return RG_OK;
-}
-
-static int copy4_in(rg_ctx* c, double* const dst[4], const double* const src[4], size_t n) {
-  for (int k = 0; k < 4; ++k)
-    if (!src[k]) return RG_E_ARG;
-  for (int k = 0; k < 4; ++k)
-    HIPCHK(c, hipMemcpyAsync(dst[k], src[k], sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return RG_OK;
-}
-
-static int copy4_out(rg_ctx* c, double* const dst[4], double* const src[4], size_t n) {
-  for (int k = 0; k < 4; ++k)
-    if (dst[k])
-      HIPCHK(c, hipMemcpyAsync(dst[k], src[k], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   return RG_OK;
 }
 
 int rg_set_cells(rg_ctx* c, const double* d, const double* p, const double* vx, const double* vy) {
   if (!ready(c, false, false)) return RG_E_ARG;
-  double* const dst[4] = {c->cells.d, c->cells.p, c->cells.vx, c->cells.vy};
   const double* const src[4] = {d, p, vx, vy};
-  const int rc = copy4_in(c, dst, src, c->N);
+  const int rc = put4(c, c->prim, src, c->N);
   if (rc == RG_OK) c->has_cells = true;
   return rc;
 }
@@ -529,16 +532,14 @@ int rg_set_cells(rg_ctx* c, const double* d, const double* p, const double* vx, 
 int rg_get_cells(rg_ctx* c, double* d, double* p, double* vx, double* vy) {
   if (!ready(c, true, false)) return RG_E_ARG;
   double* const dst[4] = {d, p, vx, vy};
-  double* const src[4] = {c->cells.d, c->cells.p, c->cells.vx, c->cells.vy};
-  return copy4_out(c, dst, src, c->N);
+  return get4(c, dst, c->prim, c->N);
 }
 
 int rg_set_extensives(rg_ctx* c, const double* m, const double* E, const double* Mx,
                       const double* My) {
   if (!ready(c, false, false)) return RG_E_ARG;
-  double* const dst[4] = {c->ext.m, c->ext.E, c->ext.Mx, c->ext.My};
   const double* const src[4] = {m, E, Mx, My};
-  const int rc = copy4_in(c, dst, src, c->N);
+  const int rc = put4(c, c->ext, src, c->N);
   if (rc == RG_OK) c->has_ext = true;
   return rc;
 }
@@ -546,16 +547,14 @@ int rg_set_extensives(rg_ctx* c, const double* m, const double* E, const double*
 int rg_get_extensives(rg_ctx* c, double* m, double* E, double* Mx, double* My) {
   if (!ready(c, false, true)) return RG_E_ARG;
   double* const dst[4] = {m, E, Mx, My};
-  double* const src[4] = {c->ext.m, c->ext.E, c->ext.Mx, c->ext.My};
-  return copy4_out(c, dst, src, c->N);
+  return get4(c, dst, c->ext, c->N);
 }
 
 int rg_set_fluxes(rg_ctx* c, const double* fm, const double* fE, const double* fMx,
                   const double* fMy) {
   if (!ready(c, false, false)) return RG_E_ARG;
-  double* const dst[4] = {c->flux.m, c->flux.E, c->flux.Mx, c->flux.My};
   const double* const src[4] = {fm, fE, fMx, fMy};
-  const int rc = copy4_in(c, dst, src, c->F);
+  const int rc = put4(c, c->flux, src, c->F);
   if (rc == RG_OK) c->has_flux = true;
   return rc;
 }
@@ -563,7 +562,7 @@ int rg_set_fluxes(rg_ctx* c, const double* fm, const double* fE, const double* f
 int rg_init_extensives(rg_ctx* c) {
   if (!ready(c, true, false)) return RG_E_ARG;
   HIPCHK(c, launch(c, KID_INIT_EXT, [&] {
-    k_init_ext<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, c->cells, c->ext, c->prm.gamma);
+    k_init_ext<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, c->prim, c->ext, c->prm.gamma);
   }));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_ext = true;
@@ -572,10 +571,7 @@ int rg_init_extensives(rg_ctx* c) {
 
 int rg_time_step(rg_ctx* c, double* dt) {
   if (!ready(c, true, false) || !dt) return RG_E_ARG;
-  HIPCHK(c, fill_words(c, 0, 1, DBL_INF_BITS));
-  HIPCHK(c, launch(c, KID_CFL, [&] {
-    k_cfl<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, c->cells, c->prm.gamma, c->words);
-  }));
+  HIPCHK(c, run_cfl(c, c->prim, 0));
   const int rc = collect_error(c);
   if (rc != RG_OK) return rc;
   double v;
@@ -586,52 +582,58 @@ int rg_time_step(rg_ctx* c, double* dt) {
 
 int rg_reconstruct(rg_ctx* c, double* slopes, double* left, double* right) {
   if (!ready(c, true, false) || c->prm.order != 2) return RG_E_ARG;
-  HIPCHK(c, run_slopes(c, c->cells, 0, 1));
+  HIPCHK(c, run_slopes(c, c->prim, 0, 1));
   if (left || right) {
-    if (!c->left.d) {
-      double* q[8];
-      for (int k = 0; k < 8; ++k) HIPCHK(c, dalloc(c, c->state_bufs, &q[k], c->F));
-      c->left = DCells{q[0], q[1], q[2], q[3]};
-      c->right = DCells{q[4], q[5], q[6], q[7]};
+    if (!c->left) {
+      HIPCHK(c, dalloc(c, &c->left, c->F));
+      HIPCHK(c, dalloc(c, &c->right, c->F));
     }
     HIPCHK(c, launch(c, KID_FACE_STATES, [&] {
-      k_face_states<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->cells, c->slopes, c->left,
+      k_face_states<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->prim, c->rec, c->left,
                                                                c->right);
     }));
   }
   const int rc = collect_error(c);
   if (rc != RG_OK) return rc;
-  if (slopes)
-    for (int k = 0; k < 8; ++k)
-      HIPCHK(c, hipMemcpyAsync(slopes + static_cast<size_t>(k) * c->N, c->slopes.s[k],
-                               sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream));
-  const DCells* lr[2] = {&c->left, &c->right};
-  double* outs[2] = {left, right};
-  for (int s = 0; s < 2; ++s) {
-    if (!outs[s]) continue;
-    double* src[4] = {lr[s]->d, lr[s]->p, lr[s]->vx, lr[s]->vy};
-    for (int k = 0; k < 4; ++k)
-      HIPCHK(c, hipMemcpyAsync(outs[s] + static_cast<size_t>(k) * c->F, src[k],
-                               sizeof(double) * c->F, hipMemcpyDeviceToHost, c->stream));
+  if (slopes) {
+    std::vector<Rec> h(static_cast<size_t>(c->N));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->rec, sizeof(Rec) * c->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = static_cast<size_t>(c->N);
+    for (size_t i = 0; i < n; ++i) {
+      const Rec& r = h[i];
+      const double v[8] = {r.xd, r.xp, r.xvx, r.xvy, r.yd, r.yp, r.yvx, r.yvy};
+      for (int k = 0; k < 8; ++k) slopes[k * n + i] = v[k];
+    }
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t F = static_cast<size_t>(c->F);
+  if (left) {
+    double* const dst[4] = {left, left + F, left + 2 * F, left + 3 * F};
+    const int r2 = get4(c, dst, c->left, c->F);
+    if (r2 != RG_OK) return r2;
+  }
+  if (right) {
+    double* const dst[4] = {right, right + F, right + 2 * F, right + 3 * F};
+    const int r2 = get4(c, dst, c->right, c->F);
+    if (r2 != RG_OK) return r2;
+  }
   return RG_OK;
 }
 
 int rg_flux(rg_ctx* c, double* fm, double* fE, double* fMx, double* fMy) {
   if (!ready(c, true, false)) return RG_E_ARG;
-  HIPCHK(c, run_flux(c, c->cells, 0, 1));
+  HIPCHK(c, run_flux(c, c->prim, 0, 1));
   const int rc = collect_error(c);
   if (rc != RG_OK) return rc;
   c->has_flux = true;
+  if (!fm && !fE && !fMx && !fMy) return RG_OK;
   double* const dst[4] = {fm, fE, fMx, fMy};
-  double* const src[4] = {c->flux.m, c->flux.E, c->flux.Mx, c->flux.My};
-  return copy4_out(c, dst, src, c->F);
+  return get4(c, dst, c->flux, c->F);
 }
 
 int rg_update_extensives(rg_ctx* c, double dt) {
   if (!ready(c, false, true) || !c->has_flux) return RG_E_ARG;
-  UpdArgs a = upd_args();
+  UpdArgs a{};
   a.in = c->ext;
   a.out = c->ext;
   a.flux = c->flux;
@@ -643,10 +645,10 @@ int rg_update_extensives(rg_ctx* c, double dt) {
 
 int rg_update_cells(rg_ctx* c) {
   if (!ready(c, false, true)) return RG_E_ARG;
-  UpdArgs a = upd_args();
+  UpdArgs a{};
   a.in = c->ext;
   a.out = c->ext;
-  a.cells = c->cells;
+  a.cells = c->prim;
   a.flags = UPD_CELLS;
   HIPCHK(c, run_update(c, a, 0, 3));
   const int rc = collect_error(c);
@@ -667,57 +669,51 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
     c->dt_hist_cap = nsteps;
   }
   // first dt: SimpleCFL on the current cells into slot 0
-  HIPCHK(c, fill_words(c, 0, 1, DBL_INF_BITS));
-  HIPCHK(c, launch(c, KID_CFL, [&] {
-    k_cfl<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, c->cells, c->prm.gamma, c->words);
-  }));
+  HIPCHK(c, run_cfl(c, c->prim, 0));
   for (int s = 0; s < nsteps; ++s) {
-    unsigned long long* cur = c->words + (s & 1);
-    unsigned long long* nxt = c->words + ((s + 1) & 1);
+    const int cur = s & 1, nxt = (s + 1) & 1;
     if (integrator == 1) {
       // hdsim::TimeAdvance: fluxes, extensive update, ZeroForce, cell update
-      HIPCHK(c, run_flux(c, c->cells, s, 1));
-      UpdArgs a = upd_args();
+      HIPCHK(c, run_flux(c, c->prim, s, 1));
+      UpdArgs a{};
       a.in = c->ext;
       a.out = c->ext;
       a.flux = c->flux;
-      a.cells = c->cells;
-      a.dt_slot = cur;
-      a.reset_slot = nullptr;
-      a.cfl_slot = nxt;
+      a.cells = c->prim;
+      a.dt_slot = c->words + cur;
+      a.partials = c->partials;
       a.dt_hist = dt_out ? c->dt_hist : nullptr;
       a.step = s;
       a.flags = UPD_FACES | UPD_ZF_AFTER | UPD_CELLS | UPD_CFL;
-      // the next slot must read +inf before this update min-reduces into it
-      HIPCHK(c, fill_words(c, (s + 1) & 1, 1, DBL_INF_BITS));
       HIPCHK(c, run_update(c, a, s, 3));
+      HIPCHK(c, run_reduce_min(c, nxt));
     } else {
       // hdsim::TimeAdvance2Heun, stage 1
-      HIPCHK(c, run_flux(c, c->cells, s, 1));
-      UpdArgs a = upd_args();
+      HIPCHK(c, run_flux(c, c->prim, s, 1));
+      UpdArgs a{};
       a.in = c->ext;
       a.out = c->mid_ext;
       a.flux = c->flux;
-      a.cells = c->mid_cells;
-      a.dt_slot = cur;
-      a.reset_slot = nxt;
+      a.cells = c->mid_prim;
+      a.dt_slot = c->words + cur;
       a.dt_hist = dt_out ? c->dt_hist : nullptr;
       a.step = s;
       a.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_CELLS;
       HIPCHK(c, run_update(c, a, s, 3));
       // stage 2 on the mid cells, then 0.5*(mid + old) and the cell update
-      HIPCHK(c, run_flux(c, c->mid_cells, s, 4));
-      UpdArgs b = upd_args();
+      HIPCHK(c, run_flux(c, c->mid_prim, s, 4));
+      UpdArgs b{};
       b.in = c->mid_ext;
       b.out = c->ext;
       b.avg = c->ext;
       b.flux = c->flux;
-      b.cells = c->cells;
-      b.dt_slot = cur;
-      b.cfl_slot = nxt;
+      b.cells = c->prim;
+      b.dt_slot = c->words + cur;
+      b.partials = c->partials;
       b.step = s;
       b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
       HIPCHK(c, run_update(c, b, s, 6));
+      HIPCHK(c, run_reduce_min(c, nxt));
     }
   }
   if (dt_out)
