@@ -497,17 +497,13 @@ __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q
   // modular_flux_calculator.cpp:56-67
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
   const double speed = (fr.nx * wx + fr.ny * wy) / fr.an;
-  const Prim pl = rotate(to_prim(lf, prm.gamma), fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
-  const Prim pr = rotate(to_prim(rt, prm.gamma), fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
-  Cons h;
-  const int rc = hllc(pl, pr, speed, h);
+  Cons c;
+  const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap, speed, c);
   D4 o{0, 0, 0, 0};
-  if (rc) {
+  if (rc)
     report(err, f, rc);
-  } else {
-    const Cons c = rotate_back(h, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap);
+  else
     o = D4{c.m, c.E, c.mx, c.my};
-  }
   flux[f] = o;
 }
 
@@ -538,8 +534,9 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q
   const int oa = fa ? a : m.gorig[a - m.N];
   const int ob = fb ? b : m.gorig[b - m.N];
   const bool mirror = oa == ob;  // GetOriginalIndex equal on both sides
-  auto reflect = [&](const Prim& s) {
-    Prim r = s;
+  // Reflect (geometry.cpp:117-120) of a side's velocity about the face
+  auto reflect = [&](const Cell& s) {
+    Cell r = s;
     const double s2 = 2 * (s.vx * px + s.vy * py);
     const double ap = vabs(px, py);
     const double a2 = ap * ap;
@@ -547,30 +544,27 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q
     r.vy = py * s2 / a2 - s.vy;
     return r;
   };
-  Prim l, r;
+  Cell l, r;
   if (!fa) {
-    r = to_prim(ld_cell(q, b), prm.gamma);
-    l = mirror ? reflect(r) : to_prim(fetch(m, q, a), prm.gamma);
+    r = ld_cell(q, b);
+    l = mirror ? reflect(r) : fetch(m, q, a);
   } else if (!fb) {
-    l = to_prim(ld_cell(q, a), prm.gamma);
-    r = mirror ? reflect(l) : to_prim(fetch(m, q, b), prm.gamma);
+    l = ld_cell(q, a);
+    r = mirror ? reflect(l) : fetch(m, q, b);
   } else {
-    l = to_prim(ld_cell(q, a), prm.gamma);
-    r = to_prim(ld_cell(q, b), prm.gamma);
+    l = ld_cell(q, a);
+    r = ld_cell(q, b);
   }
   const double an = vabs(nx, ny), ap = vabs(px, py);
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
   const double velocity = (nx * wx + ny * wy) / an;
-  Cons h;
-  const int rc = hllc(rotate(l, nx, ny, an, px, py, ap), rotate(r, nx, ny, an, px, py, ap),
-                      velocity, h);
+  Cons c;
+  const int rc = hllc_face(l, r, prm.gamma, nx, ny, an, px, py, ap, velocity, c);
   D4 o{0, 0, 0, 0};
-  if (rc) {
+  if (rc)
     report(err, f, rc);
-  } else {
-    const Cons c = rotate_back(h, nx, ny, an, px, py, ap);
+  else
     o = D4{c.m, c.E, c.mx, c.my};
-  }
   flux[f] = o;
 }
 

@@ -47,86 +47,11 @@ __device__ __forceinline__ bool is_nan_ref(double x) {
 __device__ __forceinline__ double dp2e(double g, double d, double p) { return p / d / (g - 1); }
 __device__ __forceinline__ double dp2c(double g, double d, double p) { return sqrt(g * p / d); }
 
-// convert_to_primitive (simple_flux_calculator.cpp:9-15)
-__device__ __forceinline__ Prim to_prim(const Cell& c, double g) {
-  return Prim{c.d, c.p, c.vx, c.vy, dp2e(g, c.d, c.p), dp2c(g, c.d, c.p)};
-}
-
 // TotalEnergyDensity (hydrodynamic_variables.cpp:261-265): pow(abs(v),2) is
 // evaluated as abs(v)*abs(v)
 __device__ __forceinline__ double ted(const Prim& s) {
   const double a = vabs(s.vx, s.vy);
   return s.d * (0.5 * (a * a) + s.e);
-}
-
-// HLLC in the face frame, normal along x (hllc.cpp:27-154, massflux_=true).
-// Returns 0 or an error code.
-__device__ __forceinline__ int hllc(Prim l, Prim r, double velocity, Cons& out) {
-  if (is_nan_ref(r.vx)) return ERR_HLLC_NAN;
-  // Velocity -= velocity*normaldir with normaldir = (1,0)
-  l.vx = l.vx - 1.0 * velocity;
-  l.vy = l.vy - 0.0 * velocity;
-  r.vx = r.vx - 1.0 * velocity;
-  r.vy = r.vy - 0.0 * velocity;
-  // estimate_wave_speeds (hllc.cpp:27-43)
-  const double sl = smin(l.vx - l.c, r.vx - r.c);
-  const double sr = smax(l.vx + l.c, r.vx + r.c);
-  const double ss = (r.p - l.p + l.d * l.vx * (sl - l.vx) - r.d * r.vx * (sr - r.vx)) /
-                    (l.d * (sl - l.vx) - r.d * (sr - r.vx));
-  Cons f;
-  // branch order of hllc.cpp:134-148
-  const int branch = (sl > 0) ? 0 : (sl <= 0 && ss >= 0) ? 1 : (ss < 0 && sr >= 0) ? 2
-                                                                : (sr < 0) ? 3 : 4;
-  if (branch == 0 || branch == 3) {
-    // supersonic: Primitive2Flux of the upwind side (hydrodynamic_variables.cpp:281-290)
-    const Prim& s = branch == 0 ? l : r;
-    const double un = s.vx * 1.0 + s.vy * 0.0;
-    const double dun = s.d * un;
-    f.m = s.d * un;
-    f.mx = 1.0 * s.p + s.vx * dun;
-    f.my = 0.0 * s.p + s.vy * dun;
-    f.E = (ted(s) + s.p) * un;
-  } else if (branch == 1) {
-    // fl + sl*(usl - ul)
-    const double un = l.vx * 1.0 + l.vy * 0.0;
-    const double dun = l.d * un;
-    const double e_l = ted(l);
-    const double ds = l.d * (sl - l.vx) / (sl - ss);  // starred_state, hllc.cpp:76-92
-    const double usE = e_l * ds / l.d + ds * (ss - l.vx) * (ss + l.p / l.d / (sl - l.vx));
-    f.m = l.d * un + sl * (ds - l.d);
-    f.mx = (1.0 * l.p + l.vx * dun) + (ds * ss - l.vx * l.d) * sl;
-    f.my = (0.0 * l.p + l.vy * dun) + (ds * l.vy - l.vy * l.d) * sl;
-    f.E = (e_l + l.p) * un + sl * (usE - e_l);
-  } else if (branch == 2) {
-    // fr + sr*(usr - ur)
-    const double un = r.vx * 1.0 + r.vy * 0.0;
-    const double dun = r.d * un;
-    const double e_r = ted(r);
-    const double ds = r.d * (sr - r.vx) / (sr - ss);
-    const double usE = e_r * ds / r.d + ds * (ss - r.vx) * (ss + r.p / r.d / (sr - r.vx));
-    f.m = r.d * un + sr * (ds - r.d);
-    f.mx = (1.0 * r.p + r.vx * dun) + (ds * ss - r.vx * r.d) * sr;
-    f.my = (0.0 * r.p + r.vy * dun) + (ds * r.vy - r.vy * r.d) * sr;
-    f.E = (e_r + r.p) * un + sr * (usE - e_r);
-  } else {
-    return ERR_INVALID_WAVE_SPEEDS;
-  }
-  // back to the lab frame (hllc.cpp:150-152)
-  f.E = f.E + (f.mx * (1.0 * velocity) + f.my * (0.0 * velocity) + 0.5 * f.m * velocity * velocity);
-  const double vm = velocity * f.m;
-  f.mx = f.mx + 1.0 * vm;
-  f.my = f.my + 0.0 * vm;
-  out = f;
-  return 0;
-}
-
-// rotate (simple_flux_calculator.cpp:130-140); an = |n|, ap = |p|
-__device__ __forceinline__ Prim rotate(const Prim& s, double nx, double ny, double an, double px,
-                                       double py, double ap) {
-  Prim r = s;
-  r.vx = (s.vx * nx + s.vy * ny) / an;
-  r.vy = (s.vx * px + s.vy * py) / ap;
-  return r;
 }
 
 // rotate_back (simple_flux_calculator.cpp:142-151)
@@ -153,6 +78,59 @@ __device__ __forceinline__ Cell interp(const Cell& c, const Cell& sx, const Cell
   add_mult(r, sx, tx - cmx);
   add_mult(r, sy, ty - cmy);
   return r;
+}
+
+// One face: convert_to_primitive on both sides, rotate into the face frame,
+// Hllc, rotate_back (simple_flux_calculator.cpp:9-15,130-165; hllc.cpp:97-154).
+// The HLLC branch uses only one side's energy and tangential velocity, so
+// those two quantities (three IEEE divisions) are computed for that side
+// only; every value that is computed is evaluated exactly as above.
+__device__ __forceinline__ int hllc_face(const Cell& L, const Cell& R, double g, double nx,
+                                         double ny, double an, double px, double py, double ap,
+                                         double velocity, Cons& out) {
+  const double cl = dp2c(g, L.d, L.p), cr = dp2c(g, R.d, R.p);
+  const double rvx = (R.vx * nx + R.vy * ny) / an;
+  if (is_nan_ref(rvx)) return ERR_HLLC_NAN;
+  const double lvx = (L.vx * nx + L.vy * ny) / an;
+  // Velocity -= velocity*normaldir, normaldir = (1,0)
+  const double ul = lvx - 1.0 * velocity, ur = rvx - 1.0 * velocity;
+  const double sl = smin(ul - cl, ur - cr);
+  const double sr = smax(ul + cl, ur + cr);
+  const double ss = (R.p - L.p + L.d * ul * (sl - ul) - R.d * ur * (sr - ur)) /
+                    (L.d * (sl - ul) - R.d * (sr - ur));
+  const int branch = (sl > 0) ? 0 : (sl <= 0 && ss >= 0) ? 1 : (ss < 0 && sr >= 0) ? 2
+                                                            : (sr < 0) ? 3 : 4;
+  if (branch == 4) return ERR_INVALID_WAVE_SPEEDS;
+  const bool left = branch <= 1;
+  const Cell& S = left ? L : R;
+  const double u = left ? ul : ur;
+  const double sk = left ? sl : sr;
+  // the upwind side's tangential velocity (rotate) and specific energy (dp2e)
+  const double v = (S.vx * px + S.vy * py) / ap - 0.0 * velocity;
+  const Prim s{S.d, S.p, u, v, dp2e(g, S.d, S.p), 0.0};
+  const double un = s.vx * 1.0 + s.vy * 0.0;
+  const double dun = s.d * un;
+  const double es = ted(s);
+  Cons f;
+  if (branch == 0 || branch == 3) {
+    f.m = s.d * un;
+    f.mx = 1.0 * s.p + s.vx * dun;
+    f.my = 0.0 * s.p + s.vy * dun;
+    f.E = (es + s.p) * un;
+  } else {
+    const double ds = s.d * (sk - s.vx) / (sk - ss);  // starred_state, hllc.cpp:76-92
+    const double usE = es * ds / s.d + ds * (ss - s.vx) * (ss + s.p / s.d / (sk - s.vx));
+    f.m = s.d * un + sk * (ds - s.d);
+    f.mx = (1.0 * s.p + s.vx * dun) + (ds * ss - s.vx * s.d) * sk;
+    f.my = (0.0 * s.p + s.vy * dun) + (ds * s.vy - s.vy * s.d) * sk;
+    f.E = (es + s.p) * un + sk * (usE - es);
+  }
+  f.E = f.E + (f.mx * (1.0 * velocity) + f.my * (0.0 * velocity) + 0.5 * f.m * velocity * velocity);
+  const double vm = velocity * f.m;
+  f.mx = f.mx + 1.0 * vm;
+  f.my = f.my + 0.0 * vm;
+  out = rotate_back(f, nx, ny, an, px, py, ap);
+  return 0;
 }
 
 }  // namespace rg
