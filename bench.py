@@ -50,6 +50,21 @@ def kernel_alg_bytes(name, N, F, G, nnz, moving=False):
     return 0
 
 
+def measured_traffic(kernel, nx, ny):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if
+    there is no measurement for it."""
+    path = os.path.join(ROOT, "profiles", f"kh{nx}x{ny}_fp64_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    i = text.find("```json")
+    if i < 0:
+        return None
+    block = text[i + 7:text.index("```", i + 7)]
+    return json.loads(block).get("kernels", {}).get(kernel)
+
+
 def calibrate_copy(mib):
     """One hipMemcpy device-to-device of `mib` MiB (plus one warm-up copy of
     1 MiB), through the HIP runtime: its copy kernel reads and writes exactly
@@ -177,7 +192,8 @@ def main():
         achieved = alg / avg_s / 1e9
         line["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                            "frac": achieved / HBM_PEAK_GBS,
+                            "traffic": measured_traffic(kname, args.nx, args.ny),
                             "alg_bytes_per_launch": alg, "avg_launch_us": avg_s * 1e6}
         line["kernels"] = {k: {"launches": n, "avg_us": 1e3 * ms / n,
                                "alg_GBs": kernel_alg_bytes(k, N, F, G, nnz) / (ms / n / 1e3) / 1e9}
