@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel HIP events")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision run (FP32 reconstruction and HLLC)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HIP runtime copy "
                          "kernel) and exit: the reference point for HBM byte counters")
@@ -167,6 +169,37 @@ def main():
     ext = g.get_extensives()
 
     value = N * args.steps / elapsed / 1e6
+    c64 = g.get_cells()
+
+    # Mixed precision: the same run (same initial state, warm-up and steps)
+    # with FP32 reconstruction and HLLC and FP64 extensives; L1 error of each
+    # field against the FP64 run after the same steps.
+    mixed = None
+    if not args.no_mixed:
+        from rich_amd.gpu import RG_MIXED
+        gm = RichGPU(mesh, order=2, cfl=0.3, precision=RG_MIXED)
+        gm.set_cells(cells)
+        gm.init_extensives()
+        gm.advance(args.warmup, integrator=2, record_dt=False)
+        if not args.no_profile:
+            gm.profile(True)
+        tm0 = time.perf_counter()
+        gm.advance(args.steps, integrator=2, record_dt=False)
+        elapsed_m = time.perf_counter() - tm0
+        prof_m = gm.profile_read() if not args.no_profile else {}
+        cm = gm.get_cells()
+        vm = N * args.steps / elapsed_m / 1e6
+        names = ("density", "pressure", "x_velocity", "y_velocity")
+        l1 = {k: float(np.mean(np.abs(cm[j] - c64[j]))) for j, k in enumerate(names)}
+        l1_rel = {k: float(np.sum(np.abs(cm[j] - c64[j])) / np.sum(np.abs(c64[j])))
+                  for j, k in enumerate(names)}
+        mixed = {"value": vm, "ms_per_step": 1e3 * elapsed_m / args.steps,
+                 "speedup_vs_fp64": vm / value, "l1_vs_fp64": l1,
+                 "l1_rel_vs_fp64": l1_rel, "steps_compared": args.warmup + args.steps}
+        if prof_m:
+            mixed["kernels"] = {k: {"launches": n, "avg_us": 1e3 * ms / n}
+                                for k, (n, ms) in sorted(prof_m.items())}
+        gm.close()
     line = {
         "metric": "Mcell-updates/s on 1 GPU (2D, 2M cells) FP64 vs mixed; %HBM roofline; adjoint/fwd",
         "value": value,
@@ -202,6 +235,8 @@ def main():
     stage_gbs = b_alg * 2 * args.steps / elapsed / 1e9
     line["stage_roofline"] = {"bytes_per_stage": b_alg, "achieved": stage_gbs,
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}
+    if mixed:
+        line["mixed"] = mixed
     if cpu:
         line["cpu_baseline"] = cpu
         line["speedup_vs_cpu"] = value / cpu["value"]

@@ -1,0 +1,423 @@
+// Mixed-precision step: FP32 reconstruction (LinearGaussImproved) and FP32
+// rotated HLLC; FP64 extensives, accumulated over faces with Neumaier
+// compensation, and FP64 primitive recovery (pressure from total energy).
+//
+// FP32 operands are rounded from FP64 quantities that are formed before any
+// cancellation: face-centroid offsets are taken relative to each side's
+// centre of mass in FP64, face normals and tangents are unit vectors, and
+// face area is folded into the flux, so no FP32 subtraction of absolute
+// coordinates occurs.  The FP32 kernels use the GPU's fast reciprocal and
+// square root.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace rg {
+
+struct alignas(16) F4 {
+  float a, b, c, d;
+};
+// per face: h = r_ij/2 from the first side, centroid minus each side's CM
+struct alignas(32) FaceRecM {
+  float hx, hy, oxa, oya, oxb, oyb, pad0, pad1;
+};
+// per face: unit normal n/|n|, unit tangent p/|p|, area
+struct alignas(32) FaceFrameM {
+  float nx, ny, px, py, area, pad0, pad1, pad2;
+};
+// per cell: kind, width, gradient matrix (naive) or least-squares sums (boundary)
+struct alignas(32) CellGeoM {
+  float kind, width, c0, c1, c2, c3, pad0, pad1;
+};
+// per cell record: cell, x-slope, y-slope
+struct alignas(16) RecM {
+  float d, p, vx, vy, xd, xp, xvx, xvy, yd, yp, yvx, yvy;
+};
+
+struct MTables {
+  const FaceRecM* frec;
+  const FaceFrameM* fframe;
+  const CellGeoM* cgeo;
+};
+
+struct CellF {
+  float d, p, vx, vy;
+};
+
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmaxr(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ CellF ld_cellf(const F4* q, int i) {
+  const F4 v = q[i];
+  return CellF{v.a, v.b, v.c, v.d};
+}
+
+__device__ __forceinline__ CellF fetchf(const DMesh& m, const F4* q, int id) {
+  if (id < m.N) return ld_cellf(q, id);
+  const int g = id - m.N;
+  CellF r = ld_cellf(q, m.gsrc[g]);
+  if (m.bc == BC_RIGID) {
+    const D2 n = m.gn[g];
+    const float nx = static_cast<float>(n.x), ny = static_cast<float>(n.y);
+    const float s2 = 2.f * (r.vx * nx + r.vy * ny);
+    r.vx = r.vx - nx * s2;
+    r.vy = r.vy - ny * s2;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void add_multf(CellF& r, const CellF& o, float s) {
+  r.d += o.d * s;
+  r.p += o.p * s;
+  r.vx += o.vx * s;
+  r.vy += o.vy * s;
+}
+__device__ __forceinline__ CellF subf(const CellF& a, const CellF& b) {
+  return CellF{a.d - b.d, a.p - b.p, a.vx - b.vx, a.vy - b.vy};
+}
+__device__ __forceinline__ void minmaxf(CellF& mx, CellF& mn, const CellF& t) {
+  mx.d = fmaxr(mx.d, t.d);
+  mx.p = fmaxr(mx.p, t.p);
+  mx.vx = fmaxr(mx.vx, t.vx);
+  mx.vy = fmaxr(mx.vy, t.vy);
+  mn.d = fminr(mn.d, t.d);
+  mn.p = fminr(mn.p, t.p);
+  mn.vx = fminr(mn.vx, t.vx);
+  mn.vy = fminr(mn.vy, t.vy);
+}
+__device__ __forceinline__ void limit_fieldf(float dphi, float cval, float cell, float maxd,
+                                             float mind, bool absguard, float& psi) {
+  if (fabsf(dphi) > 0.05f * fmaxr(fabsf(maxd), fabsf(mind)) || cval * cell < 0) {
+    const float hi = absguard ? fabsf(1e-9f * cell) : 1e-9f * cell;
+    const float lo = -hi;
+    if (dphi > hi)
+      psi = fminr(psi, fdiv(maxd, dphi));
+    else if (dphi < lo)
+      psi = fminr(psi, fdiv(mind, dphi));
+  }
+}
+__device__ __forceinline__ void shocked_fieldf(float dphi, float cval, float cell, float nval,
+                                               float maxd, float mind, float theta, float& psi) {
+  if (fabsf(dphi) > 0.1f * fmaxr(fabsf(maxd), fabsf(mind)) || cval * cell < 0) {
+    if (fabsf(dphi) > 1e-9f * cell) psi = fminr(psi, fmaxr(fdiv(theta * (nval - cell), dphi), 0.f));
+  }
+}
+
+// Static FP32 tables, formed in FP64 and rounded once.
+__global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceRecM* frec,
+                                                         FaceFrameM* fframe) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  const FaceRec r = m.frec[f];
+  const FaceFrame fr = m.fframe[f];
+  const D2 ca = m.cm[m.ff[f]], cb = m.cm[m.fs[f]];
+  const float oxa = static_cast<float>(r.cx - ca.x), oya = static_cast<float>(r.cy - ca.y);
+  const float oxb = static_cast<float>(r.cx - cb.x), oyb = static_cast<float>(r.cy - cb.y);
+  frec[f] = FaceRecM{static_cast<float>(r.hx), static_cast<float>(r.hy), oxa, oya, oxb, oyb, 0.f,
+                     0.f};
+  fframe[f] = FaceFrameM{static_cast<float>(fr.nx / fr.an), static_cast<float>(fr.ny / fr.an),
+                         static_cast<float>(fr.px / fr.ap), static_cast<float>(fr.py / fr.ap),
+                         static_cast<float>(m.area[f]), 0.f, 0.f, 0.f};
+}
+
+__global__ void __launch_bounds__(256) k_geo_mixed_cells(DMesh m, CellGeoM* cg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const CellGeo g = m.cgeo[i];
+  cg[i] = CellGeoM{static_cast<float>(g.kind), static_cast<float>(g.width),
+                   static_cast<float>(g.c0), static_cast<float>(g.c1), static_cast<float>(g.c2),
+                   static_cast<float>(g.c3), 0.f, 0.f};
+}
+
+// LinearGaussImproved in FP32.
+__global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const F4* __restrict__ q,
+                                                      RecM* __restrict__ out, DParams prm,
+                                                      DErr err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const int b = m.off[i], e = m.off[i + 1];
+  const CellF cell = ld_cellf(q, i);
+  const CellGeoM g = t.cgeo[i];
+  const int kind = static_cast<int>(g.kind);
+  CellF sx{0, 0, 0, 0}, sy{0, 0, 0, 0};
+  float pr = 1.f;
+  CellF amax = cell, amin = cell, bmax = cell, bmin = cell;
+  if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
+    report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
+    out[i] = RecM{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0};
+    return;
+  }
+  if (kind == KIND_BOUNDARY) {
+    CellF phy{0, 0, 0, 0}, phx{0, 0, 0, 0};
+    const D2 ci = m.cm[i];
+    for (int k = b; k < e; ++k) {
+      const Inc ic = m.inc[k];
+      const CellF nc = fetchf(m, q, ic.nbr);
+      const D2 cj = m.cm[ic.nbr];
+      const float ey = static_cast<float>(cj.y - ci.y), ex = static_cast<float>(cj.x - ci.x);
+      const CellF dd = subf(nc, cell);
+      add_multf(phy, dd, ey);
+      add_multf(phx, dd, ex);
+      pr = (cell.p > nc.p) ? fminr(pr, fdiv(nc.p, cell.p)) : fminr(pr, fdiv(cell.p, nc.p));
+      minmaxf(amax, amin, nc);
+      if (!(ic.info & INFO_MIRROR)) minmaxf(bmax, bmin, nc);
+    }
+    const float SxSy = g.c0, Sx2 = g.c1, Sy2 = g.c2, rden = __builtin_amdgcn_rcpf(g.c3);
+    sx = CellF{(phy.d * SxSy - phx.d * Sy2) * rden, (phy.p * SxSy - phx.p * Sy2) * rden,
+               (phy.vx * SxSy - phx.vx * Sy2) * rden, (phy.vy * SxSy - phx.vy * Sy2) * rden};
+    sy = CellF{(phx.d * SxSy - phy.d * Sx2) * rden, (phx.p * SxSy - phy.p * Sx2) * rden,
+               (phx.vx * SxSy - phy.vx * Sx2) * rden, (phx.vy * SxSy - phy.vy * Sx2) * rden};
+  } else {
+    CellF vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
+    for (int k0 = b; k0 < e; k0 += CH) {
+      Inc ic[CH];
+      float2 hh[CH];
+      CellF nc[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < e) ic[u] = m.inc[k0 + u];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < e) {
+          const FaceRecM fr = t.frec[ic[u].info & INFO_FACE];
+          hh[u] = (ic[u].info & INFO_SECOND) ? make_float2(-fr.hx, -fr.hy) : make_float2(fr.hx, fr.hy);
+          nc[u] = fetchf(m, q, ic[u].nbr);
+        }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (k0 + u >= e) break;
+        add_multf(vcx, cell, hh[u].x);
+        add_multf(vcy, cell, hh[u].y);
+        add_multf(vcy, nc[u], hh[u].y);
+        add_multf(vcx, nc[u], hh[u].x);
+        pr = (cell.p > nc[u].p) ? fminr(pr, fdiv(nc[u].p, cell.p)) : fminr(pr, fdiv(cell.p, nc[u].p));
+        minmaxf(amax, amin, nc[u]);
+      }
+    }
+    bmax = amax;
+    bmin = amin;
+    sx = CellF{vcx.d * g.c0, vcx.p * g.c0, vcx.vx * g.c0, vcx.vy * g.c0};
+    sy = CellF{vcy.d * g.c3, vcy.p * g.c3, vcy.vx * g.c3, vcy.vy * g.c3};
+    add_multf(sx, vcy, g.c1);
+    add_multf(sy, vcx, g.c2);
+  }
+  if (prm.slf) {
+    const float cs = fsqrt(fdiv(static_cast<float>(prm.gamma) * cell.p, cell.d));
+    const bool shock = ((sx.vx + sy.vy) * g.width < static_cast<float>(-prm.delta_v) * cs) ||
+                       (pr < static_cast<float>(prm.delta_p));
+    float psi0 = 1, psi1 = 1, psi2 = 1, psi3 = 1;
+    const CellF maxd = subf(shock ? amax : bmax, cell), mind = subf(shock ? amin : bmin, cell);
+    const float theta = static_cast<float>(prm.theta);
+    for (int k = b; k < e; ++k) {
+      const Inc ic = m.inc[k];
+      const FaceRecM fr = t.frec[ic.info & INFO_FACE];
+      const float ox = (ic.info & INFO_SECOND) ? fr.oxb : fr.oxa;
+      const float oy = (ic.info & INFO_SECOND) ? fr.oyb : fr.oya;
+      CellF cv = cell;
+      add_multf(cv, sx, ox);
+      add_multf(cv, sy, oy);
+      const CellF dphi = subf(cv, cell);
+      if (!shock) {
+        limit_fieldf(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
+        limit_fieldf(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
+        limit_fieldf(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
+        limit_fieldf(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
+      } else {
+        const CellF nc = fetchf(m, q, ic.nbr);
+        shocked_fieldf(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, theta, psi0);
+        shocked_fieldf(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, theta, psi1);
+        shocked_fieldf(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, theta, psi2);
+        shocked_fieldf(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, theta, psi3);
+      }
+    }
+    sx.d *= psi0;
+    sy.d *= psi0;
+    sx.p *= psi1;
+    sy.p *= psi1;
+    sx.vx *= psi2;
+    sy.vx *= psi2;
+    sx.vy *= psi3;
+    sy.vy *= psi3;
+  }
+  out[i] = RecM{cell.d, cell.p, cell.vx, cell.vy, sx.d, sx.p, sx.vx, sx.vy,
+                sy.d,   sy.p,   sy.vx,   sy.vy};
+}
+
+// face state of one side in FP32; (ox, oy) = centroid - CM of that side
+__device__ __forceinline__ CellF side_state_m(const DMesh& m, const F4* q, const RecM* rec, int id,
+                                              float ox, float oy) {
+  if (id >= m.N && m.bc == BC_RIGID) return fetchf(m, q, id);  // zero ghost gradient
+  const RecM r = rec[id < m.N ? id : m.gorig[id - m.N]];
+  CellF v{r.d, r.p, r.vx, r.vy};
+  add_multf(v, CellF{r.xd, r.xp, r.xvx, r.xvy}, ox);
+  add_multf(v, CellF{r.yd, r.yp, r.yvx, r.yvy}, oy);
+  return v;
+}
+
+// rotated HLLC in FP32 (same wave-speed estimate and star states as the FP64
+// path); returns the lab-frame flux times the face area
+__device__ __forceinline__ int hllc_face_m(const CellF& L, const CellF& R, float g, float nx,
+                                           float ny, float px, float py, float area,
+                                           float velocity, F4& out) {
+  const float cl = fsqrt(fdiv(g * L.p, L.d)), cr = fsqrt(fdiv(g * R.p, R.d));
+  const float rvx = R.vx * nx + R.vy * ny;
+  if (!(rvx == rvx)) return ERR_HLLC_NAN;
+  const float lvx = L.vx * nx + L.vy * ny;
+  const float ul = lvx - velocity, ur = rvx - velocity;
+  const float sl = fminr(ul - cl, ur - cr);
+  const float sr = fmaxr(ul + cl, ur + cr);
+  const float ss = fdiv(R.p - L.p + L.d * ul * (sl - ul) - R.d * ur * (sr - ur),
+                        L.d * (sl - ul) - R.d * (sr - ur));
+  const int branch = (sl > 0) ? 0 : (sl <= 0 && ss >= 0) ? 1 : (ss < 0 && sr >= 0) ? 2
+                                                            : (sr < 0) ? 3 : 4;
+  if (branch == 4) return ERR_INVALID_WAVE_SPEEDS;
+  const bool left = branch <= 1;
+  const CellF& S = left ? L : R;
+  const float u = left ? ul : ur;
+  const float sk = left ? sl : sr;
+  const float v = S.vx * px + S.vy * py;
+  const float rd = __builtin_amdgcn_rcpf(S.d);
+  const float e = S.p * rd * __builtin_amdgcn_rcpf(g - 1.f);
+  const float es = S.d * (0.5f * (u * u + v * v) + e);
+  const float dun = S.d * u;
+  float fm, fx, fy, fE;
+  if (branch == 0 || branch == 3) {
+    fm = dun;
+    fx = S.p + u * dun;
+    fy = v * dun;
+    fE = (es + S.p) * u;
+  } else {
+    const float ds = fdiv(S.d * (sk - u), sk - ss);
+    const float usE = es * ds * rd + ds * (ss - u) * (ss + fdiv(S.p * rd, sk - u));
+    fm = dun + sk * (ds - S.d);
+    fx = (S.p + u * dun) + (ds * ss - u * S.d) * sk;
+    fy = v * dun + (ds * v - v * S.d) * sk;
+    fE = (es + S.p) * u + sk * (usE - es);
+  }
+  fE = fE + (fx * velocity + 0.5f * fm * velocity * velocity);
+  fx = fx + velocity * fm;
+  // rotate back with unit vectors, times the face area
+  out = F4{fm * area, fE * area, (fx * nx + fy * px) * area, (fx * ny + fy * py) * area};
+  return 0;
+}
+
+__global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4* __restrict__ q,
+                                                    const RecM* __restrict__ rec,
+                                                    F4* __restrict__ flux, DParams prm, DErr err) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  const FaceFrameM fr = t.fframe[f];
+  const FaceRecM fo = t.frec[f];
+  const CellF lt = side_state_m(m, q, rec, m.ff[f], fo.oxa, fo.oya);
+  const CellF rt = side_state_m(m, q, rec, m.fs[f], fo.oxb, fo.oyb);
+  const float wx = m.wx ? static_cast<float>(m.wx[f]) : 0.f;
+  const float wy = m.wy ? static_cast<float>(m.wy[f]) : 0.f;
+  const float speed = fr.nx * wx + fr.ny * wy;
+  F4 o{0, 0, 0, 0};
+  const int rc = hllc_face_m(lt, rt, static_cast<float>(prm.gamma), fr.nx, fr.ny, fr.px, fr.py,
+                             fr.area, speed, o);
+  if (rc) {
+    report(err, f, rc);
+    o = F4{0, 0, 0, 0};
+  }
+  flux[f] = o;
+}
+
+// Neumaier-compensated FP64 sum
+struct KSum {
+  double s, c;
+  __device__ __forceinline__ void add(double x) {
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+struct UpdArgsM {
+  // per CSR entry: face index | INFO_SECOND.  A periodic boundary face and
+  // its translated copy on the other side of the box are both read from the
+  // lower-numbered copy, so the two cells see one FP32 flux with opposite
+  // signs and the totals telescope.
+  const unsigned* info;
+  const D4* in;
+  D4* out;
+  const D4* avg;
+  const F4* flux;  // flux times area
+  F4* cells;       // FP32 copy for the next reconstruction
+  const unsigned long long* dt_slot;
+  unsigned long long* reset_slot;
+  double* partials;
+  double* dt_hist;
+  int step;
+  int flags;
+};
+
+__global__ void __launch_bounds__(256) k_update_mixed(DMesh m, UpdArgsM a, DParams prm, DErr err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
+  if (i == 0) {
+    if (a.dt_hist) a.dt_hist[a.step] = dt;
+    if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
+  }
+  double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  if (i < m.N) {
+    const D4 x = a.in[i];
+    KSum sm{x.a, 0}, sE{x.b, 0}, sx{x.c, 0}, sy{x.d, 0};
+    const int kb = m.off[i], ke = m.off[i + 1];
+    for (int k0 = kb; k0 < ke; k0 += CH) {
+      unsigned inf[CH];
+      F4 fl[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < ke) inf[u] = a.info[k0 + u];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (k0 + u < ke) fl[u] = a.flux[inf[u] & INFO_FACE];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (k0 + u >= ke) break;
+        const double s = (inf[u] & INFO_SECOND) ? dt : -dt;
+        sm.add(s * static_cast<double>(fl[u].a));
+        sE.add(s * static_cast<double>(fl[u].b));
+        sx.add(s * static_cast<double>(fl[u].c));
+        sy.add(s * static_cast<double>(fl[u].d));
+      }
+    }
+    double em = sm.value(), eE = sE.value(), ex = sx.value(), ey = sy.value();
+    if (a.flags & UPD_AVERAGE) {
+      const D4 o = a.avg[i];
+      em = 0.5 * (em + o.a);
+      eE = 0.5 * (eE + o.b);
+      ex = (ex + o.c) * 0.5;
+      ey = (ey + o.d) * 0.5;
+    }
+    a.out[i] = D4{em, eE, ex, ey};
+    // FP64 primitive recovery
+    const double d = em / m.vol[i];
+    const double vx = ex / em, vy = ey / em;
+    const double e = eE / em - 0.5 * (vx * vx + vy * vy);
+    if (d < 0)
+      report(err, i, ERR_NEGATIVE_DENSITY);
+    else if (e < 0)
+      report(err, i, ERR_NEGATIVE_THERMAL);
+    const double p = (prm.gamma - 1) * e * d;
+    a.cells[i] = F4{static_cast<float>(d), static_cast<float>(p), static_cast<float>(vx),
+                    static_cast<float>(vy)};
+    if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, m.cgeo[i].width, Cell{d, p, vx, vy});
+  }
+  if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
+}
+
+__global__ void k_to_f32(const D4* __restrict__ s, F4* __restrict__ d, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const D4 v = s[i];
+    d[i] = F4{static_cast<float>(v.a), static_cast<float>(v.b), static_cast<float>(v.c),
+              static_cast<float>(v.d)};
+  }
+}
+
+}  // namespace rg

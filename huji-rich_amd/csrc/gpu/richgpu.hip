@@ -10,9 +10,10 @@
 #include <cstring>
 #include <new>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
-#include "kernels.hpp"
+#include "kernels_mixed.hpp"
 
 using namespace rg;
 
@@ -30,12 +31,16 @@ enum KernelId : int {
   KID_GEO,
   KID_PACK,
   KID_REDUCE,
+  KID_SLOPES_MIXED,
+  KID_FLUX_MIXED,
+  KID_UPDATE_MIXED,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
                                              "k_flux1",       "k_update",   "k_face_states",
                                              "k_init_ext",    "k_fill_u64", "k_geo",
-                                             "k_pack",        "k_reduce_min"};
+                                             "k_pack",        "k_reduce_min", "k_slopes_mixed",
+                                             "k_flux_mixed",  "k_update_mixed"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -81,6 +86,12 @@ struct rg_ctx {
   unsigned long long* host_words = nullptr;
   double* dt_hist = nullptr;
   int dt_hist_cap = 0;
+
+  // mixed precision (rg_advance with RG_MIXED)
+  MTables mt{};
+  F4 *primM = nullptr, *mid_primM = nullptr, *fluxM = nullptr;
+  RecM* recM = nullptr;
+  unsigned* minfo = nullptr;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -135,6 +146,10 @@ void free_mesh(rg_ctx* c) {
   c->prim = c->mid_prim = c->ext = c->mid_ext = c->flux = c->left = c->right = nullptr;
   c->rec = nullptr;
   c->staging = c->partials = c->wbuf = nullptr;
+  c->mt = MTables{};
+  c->primM = c->mid_primM = c->fluxM = nullptr;
+  c->minfo = nullptr;
+  c->recM = nullptr;
   c->has_mesh = c->has_cells = c->has_ext = c->has_flux = false;
 }
 
@@ -288,6 +303,67 @@ int get4(rg_ctx* c, double* const dst[4], const D4* src, int n) {
                                hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RG_OK;
+}
+
+
+// hdsim::TimeAdvance2Heun with FP32 reconstruction and HLLC and FP64
+// extensives (RG_MIXED).  The FP64 cells are recovered at the end.
+int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
+  const int N = c->N, F = c->F;
+  HIPCHK(c, launch(c, KID_PACK, [&] {
+    k_to_f32<<<blocks_for(N), kBlock, 0, c->stream>>>(c->prim, c->primM, N);
+  }));
+  HIPCHK(c, run_cfl(c, c->prim, 0));
+  for (int s = 0; s < nsteps; ++s) {
+    const int cur = s & 1, nxt = (s + 1) & 1;
+    for (int stage = 0; stage < 2; ++stage) {
+      const F4* q = stage == 0 ? c->primM : c->mid_primM;
+      HIPCHK(c, launch(c, KID_SLOPES_MIXED, [&] {
+        k_slopes_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, q, c->recM,
+                                                                dparams(c), derr(c, s, 1 + 3 * stage));
+      }));
+      HIPCHK(c, launch(c, KID_FLUX_MIXED, [&] {
+        k_flux_mixed<<<blocks_for(F), kBlock, 0, c->stream>>>(c->dm, c->mt, q, c->recM, c->fluxM,
+                                                              dparams(c), derr(c, s, 2 + 3 * stage));
+      }));
+      UpdArgsM a{};
+      a.flux = c->fluxM;
+      a.info = c->minfo;
+      a.dt_slot = c->words + cur;
+      a.step = s;
+      if (stage == 0) {
+        a.in = c->ext;
+        a.out = c->mid_ext;
+        a.cells = c->mid_primM;
+        a.reset_slot = c->words + nxt;
+        a.dt_hist = dt_out ? c->dt_hist : nullptr;
+        a.flags = 0;
+      } else {
+        a.in = c->mid_ext;
+        a.out = c->ext;
+        a.avg = c->ext;
+        a.cells = c->primM;
+        a.partials = c->partials;
+        a.flags = UPD_AVERAGE | UPD_CFL;
+      }
+      HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
+        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c),
+                                                                derr(c, s, 3 + 3 * stage));
+      }));
+    }
+    HIPCHK(c, run_reduce_min(c, nxt));
+  }
+  // FP64 cells from the FP64 extensives
+  UpdArgs u{};
+  u.in = c->ext;
+  u.out = c->ext;
+  u.cells = c->prim;
+  u.flags = UPD_CELLS;
+  HIPCHK(c, run_update(c, u, nsteps, 6));
+  if (dt_out)
+    HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
+                             c->stream));
+  return collect_error(c);
 }
 
 }  // namespace
@@ -496,6 +572,70 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, dalloc(c, &c->staging, 4 * static_cast<size_t>(std::max(N, F))));
   HIPCHK(c, dalloc(c, &c->partials, blocks_for(std::max(N, F))));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->prm.precision == RG_MIXED) {
+    FaceRecM* frecM;
+    FaceFrameM* fframeM;
+    CellGeoM* cgeoM;
+    HIPCHK(c, dalloc(c, &frecM, F));
+    HIPCHK(c, dalloc(c, &fframeM, F));
+    HIPCHK(c, dalloc(c, &cgeoM, N));
+    HIPCHK(c, dalloc(c, &c->primM, N));
+    HIPCHK(c, dalloc(c, &c->mid_primM, N));
+    HIPCHK(c, dalloc(c, &c->fluxM, F));
+    HIPCHK(c, dalloc(c, &c->recM, N));
+    // Incidence words of the FP32 update.  Periodic copies of a boundary
+    // face: the real cell r of face f sees ghost g (a translate of cell b by
+    // shift q); the copy of f on b's side is the face between b and a ghost
+    // of r translated by -q.  Both cells read the lower-numbered copy.
+    std::vector<unsigned> minfo(static_cast<size_t>(std::max(nnz, 1)));
+    for (int k = 0; k < nnz; ++k) minfo[k] = inc[k].info & (INFO_FACE | INFO_SECOND);
+    if (c->prm.bc == RG_BC_PERIODIC && G > 0) {
+      double x0 = m->pt_x[0], x1 = x0, y0 = m->pt_y[0], y1 = y0;
+      for (int i = 1; i < N; ++i) {
+        x0 = std::min(x0, m->pt_x[i]); x1 = std::max(x1, m->pt_x[i]);
+        y0 = std::min(y0, m->pt_y[i]); y1 = std::max(y1, m->pt_y[i]);
+      }
+      const double hw = 0.5 * (x1 - x0), hh = 0.5 * (y1 - y0);
+      auto key = [](int r, int b, int qx, int qy) {
+        return (static_cast<unsigned long long>(r) << 32 | static_cast<unsigned>(b)) * 9u +
+               static_cast<unsigned long long>((qx + 1) * 3 + (qy + 1));
+      };
+      std::unordered_map<unsigned long long, int> bface;
+      std::vector<unsigned long long> twin_key(static_cast<size_t>(F), ~0ull);
+      for (int f = 0; f < F; ++f) {
+        const int a = m->face_first[f], b = m->face_second[f];
+        const int g = a >= N ? a : (b >= N ? b : -1);
+        if (g < 0) continue;
+        const int r = g == a ? b : a, o = orig(g);
+        if (o == r) continue;
+        const double sx = m->pt_x[g] - m->pt_x[o], sy = m->pt_y[g] - m->pt_y[o];
+        const int qx = sx > hw ? 1 : (sx < -hw ? -1 : 0);
+        const int qy = sy > hh ? 1 : (sy < -hh ? -1 : 0);
+        bface.emplace(key(r, o, qx, qy), f);
+        twin_key[f] = key(o, r, -qx, -qy);
+      }
+      for (int i = 0; i < N; ++i)
+        for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
+          const int f = static_cast<int>(inc[k].info & INFO_FACE);
+          if (twin_key[f] == ~0ull) continue;
+          const auto it = bface.find(twin_key[f]);
+          if (it == bface.end() || it->second >= f) continue;
+          const int t = it->second;  // cell i is the ghost side of face t
+          minfo[k] = static_cast<unsigned>(t) | (m->face_second[t] >= N ? INFO_SECOND : 0u);
+        }
+    }
+    unsigned* dinfo;
+    HIPCHK(c, upload(c, &dinfo, minfo.data(), minfo.size()));  // synchronised below
+    HIPCHK(c, launch(c, KID_GEO, [&] {
+      k_geo_mixed_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, frecM, fframeM);
+    }));
+    HIPCHK(c, launch(c, KID_GEO, [&] {
+      k_geo_mixed_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeoM);
+    }));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mt = MTables{frecM, fframeM, cgeoM};
+    c->minfo = dinfo;
+  }
   c->dm = d;
   c->N = N;
   c->F = F;
@@ -668,6 +808,8 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
     HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->dt_hist), sizeof(double) * nsteps));
     c->dt_hist_cap = nsteps;
   }
+  if (c->prm.precision == RG_MIXED && integrator == 2 && c->prm.order == 2)
+    return advance_mixed(c, nsteps, dt_out);
   // first dt: SimpleCFL on the current cells into slot 0
   HIPCHK(c, run_cfl(c, c->prim, 0));
   for (int s = 0; s < nsteps; ++s) {
