@@ -17,13 +17,12 @@ namespace rg {
 struct alignas(16) F4 {
   float a, b, c, d;
 };
-// per face: h = r_ij/2 from the first side, centroid minus each side's CM
-struct alignas(32) FaceRecM {
-  float hx, hy, oxa, oya, oxb, oyb, pad0, pad1;
-};
-// per face: unit normal n/|n|, unit tangent p/|p|, area
-struct alignas(32) FaceFrameM {
-  float nx, ny, px, py, area, pad0, pad1, pad2;
+// per face: unit normal n/|n|, area, and the face centroid minus each side's
+// CM.  The unit tangent is the normal turned by +90 degrees: a Voronoi face
+// is perpendicular to the segment joining its two mesh points, and the sign
+// of the tangent cancels between rotate and rotate_back.
+struct alignas(32) FaceM {
+  float nx, ny, area, pad, oxa, oya, oxb, oyb;
 };
 // per cell: kind, width, gradient matrix (naive) or least-squares sums (boundary)
 struct alignas(32) CellGeoM {
@@ -34,11 +33,22 @@ struct alignas(16) RecM {
   float d, p, vx, vy, xd, xp, xvx, xvy, yd, yp, yvx, yvy;
 };
 
+// Incidences in wavefront-sliced ELL order: the cells of wavefront w are
+// 64w..64w+63, its slot s for lane l is entry wofs[w] + 64 s + l, and it has
+// (wofs[w+1] - wofs[w]) / 64 slots (the largest face count among its cells).
+// A load of one slot across the wavefront reads contiguous memory.
 struct MTables {
-  const FaceRecM* frec;
-  const FaceFrameM* fframe;
+  const FaceM* face;
   const CellGeoM* cgeo;
+  const int* wofs;
+  const int* enbr;       // neighbour id (ghost ids as in the mesh)
+  const F4* egeo;        // (hx, hy) = r_ij/2 seen from the cell, (ox, oy) = centroid - cell CM
+  const unsigned* einfo; // flux to read: face | INFO_SECOND, periodic copies shared
 };
+
+__device__ __forceinline__ int ell_base(const MTables& t, int i) {
+  return t.wofs[i >> 6] + (i & 63);
+}
 
 struct CellF {
   float d, p, vx, vy;
@@ -106,20 +116,33 @@ __device__ __forceinline__ void shocked_fieldf(float dphi, float cval, float cel
 }
 
 // Static FP32 tables, formed in FP64 and rounded once.
-__global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceRecM* frec,
-                                                         FaceFrameM* fframe) {
+__global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceRec r = m.frec[f];
   const FaceFrame fr = m.fframe[f];
   const D2 ca = m.cm[m.ff[f]], cb = m.cm[m.fs[f]];
-  const float oxa = static_cast<float>(r.cx - ca.x), oya = static_cast<float>(r.cy - ca.y);
-  const float oxb = static_cast<float>(r.cx - cb.x), oyb = static_cast<float>(r.cy - cb.y);
-  frec[f] = FaceRecM{static_cast<float>(r.hx), static_cast<float>(r.hy), oxa, oya, oxb, oyb, 0.f,
-                     0.f};
-  fframe[f] = FaceFrameM{static_cast<float>(fr.nx / fr.an), static_cast<float>(fr.ny / fr.an),
-                         static_cast<float>(fr.px / fr.ap), static_cast<float>(fr.py / fr.ap),
-                         static_cast<float>(m.area[f]), 0.f, 0.f, 0.f};
+  face[f] = FaceM{static_cast<float>(fr.nx / fr.an), static_cast<float>(fr.ny / fr.an),
+                  static_cast<float>(m.area[f]), 0.f,
+                  static_cast<float>(r.cx - ca.x), static_cast<float>(r.cy - ca.y),
+                  static_cast<float>(r.cx - cb.x), static_cast<float>(r.cy - cb.y)};
+}
+
+// ELL geometry of every incidence: one thread per cell over its faces.
+__global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, const int* __restrict__ wofs,
+                                                       F4* __restrict__ egeo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const D2 ci = m.cm[i];
+  const int base = wofs[i >> 6] + (i & 63);
+  for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
+    const unsigned info = m.inc[k].info;
+    const FaceRec r = m.frec[info & INFO_FACE];
+    const bool second = (info & INFO_SECOND) != 0;
+    egeo[base + 64 * (k - m.off[i])] =
+        F4{static_cast<float>(second ? -r.hx : r.hx), static_cast<float>(second ? -r.hy : r.hy),
+           static_cast<float>(r.cx - ci.x), static_cast<float>(r.cy - ci.y)};
+  }
 }
 
 __global__ void __launch_bounds__(256) k_geo_mixed_cells(DMesh m, CellGeoM* cg) {
@@ -137,7 +160,8 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
                                                       DErr err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
-  const int b = m.off[i], e = m.off[i + 1];
+  const int b = m.off[i], e = m.off[i + 1], deg = e - b;
+  const int eb = ell_base(t, i);
   const CellF cell = ld_cellf(q, i);
   const CellGeoM g = t.cgeo[i];
   const int kind = static_cast<int>(g.kind);
@@ -171,23 +195,23 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
                (phx.vx * SxSy - phy.vx * Sx2) * rden, (phx.vy * SxSy - phy.vy * Sx2) * rden};
   } else {
     CellF vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
-    for (int k0 = b; k0 < e; k0 += CH) {
-      Inc ic[CH];
-      float2 hh[CH];
+    for (int s0 = 0; s0 < deg; s0 += CH) {
+      float4 hh[CH];
       CellF nc[CH];
+      int nb[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
-        if (k0 + u < e) ic[u] = m.inc[k0 + u];
-#pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (k0 + u < e) {
-          const FaceRecM fr = t.frec[ic[u].info & INFO_FACE];
-          hh[u] = (ic[u].info & INFO_SECOND) ? make_float2(-fr.hx, -fr.hy) : make_float2(fr.hx, fr.hy);
-          nc[u] = fetchf(m, q, ic[u].nbr);
+        if (s0 + u < deg) {
+          nb[u] = t.enbr[eb + 64 * (s0 + u)];
+          const F4 gg = t.egeo[eb + 64 * (s0 + u)];
+          hh[u] = make_float4(gg.a, gg.b, gg.c, gg.d);
         }
 #pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (s0 + u < deg) nc[u] = fetchf(m, q, nb[u]);
+#pragma unroll
       for (int u = 0; u < CH; ++u) {
-        if (k0 + u >= e) break;
+        if (s0 + u >= deg) break;
         add_multf(vcx, cell, hh[u].x);
         add_multf(vcy, cell, hh[u].y);
         add_multf(vcy, nc[u], hh[u].y);
@@ -210,26 +234,33 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
     float psi0 = 1, psi1 = 1, psi2 = 1, psi3 = 1;
     const CellF maxd = subf(shock ? amax : bmax, cell), mind = subf(shock ? amin : bmin, cell);
     const float theta = static_cast<float>(prm.theta);
-    for (int k = b; k < e; ++k) {
-      const Inc ic = m.inc[k];
-      const FaceRecM fr = t.frec[ic.info & INFO_FACE];
-      const float ox = (ic.info & INFO_SECOND) ? fr.oxb : fr.oxa;
-      const float oy = (ic.info & INFO_SECOND) ? fr.oyb : fr.oya;
-      CellF cv = cell;
-      add_multf(cv, sx, ox);
-      add_multf(cv, sy, oy);
-      const CellF dphi = subf(cv, cell);
-      if (!shock) {
-        limit_fieldf(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
-        limit_fieldf(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
-        limit_fieldf(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
-        limit_fieldf(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
-      } else {
-        const CellF nc = fetchf(m, q, ic.nbr);
-        shocked_fieldf(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, theta, psi0);
-        shocked_fieldf(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, theta, psi1);
-        shocked_fieldf(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, theta, psi2);
-        shocked_fieldf(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, theta, psi3);
+    for (int s0 = 0; s0 < deg; s0 += CH) {
+      float2 oo[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (s0 + u < deg) {
+          const F4 gg = t.egeo[eb + 64 * (s0 + u)];
+          oo[u] = make_float2(gg.c, gg.d);
+        }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (s0 + u >= deg) break;
+        CellF cv = cell;
+        add_multf(cv, sx, oo[u].x);
+        add_multf(cv, sy, oo[u].y);
+        const CellF dphi = subf(cv, cell);
+        if (!shock) {
+          limit_fieldf(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
+          limit_fieldf(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
+          limit_fieldf(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
+          limit_fieldf(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
+        } else {
+          const CellF nc = fetchf(m, q, t.enbr[eb + 64 * (s0 + u)]);
+          shocked_fieldf(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, theta, psi0);
+          shocked_fieldf(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, theta, psi1);
+          shocked_fieldf(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, theta, psi2);
+          shocked_fieldf(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, theta, psi3);
+        }
       }
     }
     sx.d *= psi0;
@@ -308,15 +339,14 @@ __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4
                                                     F4* __restrict__ flux, DParams prm, DErr err) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
-  const FaceFrameM fr = t.fframe[f];
-  const FaceRecM fo = t.frec[f];
-  const CellF lt = side_state_m(m, q, rec, m.ff[f], fo.oxa, fo.oya);
-  const CellF rt = side_state_m(m, q, rec, m.fs[f], fo.oxb, fo.oyb);
+  const FaceM fr = t.face[f];
+  const CellF lt = side_state_m(m, q, rec, m.ff[f], fr.oxa, fr.oya);
+  const CellF rt = side_state_m(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
   const float wx = m.wx ? static_cast<float>(m.wx[f]) : 0.f;
   const float wy = m.wy ? static_cast<float>(m.wy[f]) : 0.f;
   const float speed = fr.nx * wx + fr.ny * wy;
   F4 o{0, 0, 0, 0};
-  const int rc = hllc_face_m(lt, rt, static_cast<float>(prm.gamma), fr.nx, fr.ny, fr.px, fr.py,
+  const int rc = hllc_face_m(lt, rt, static_cast<float>(prm.gamma), fr.nx, fr.ny, -fr.ny, fr.nx,
                              fr.area, speed, o);
   if (rc) {
     report(err, f, rc);
@@ -337,11 +367,6 @@ struct KSum {
 };
 
 struct UpdArgsM {
-  // per CSR entry: face index | INFO_SECOND.  A periodic boundary face and
-  // its translated copy on the other side of the box are both read from the
-  // lower-numbered copy, so the two cells see one FP32 flux with opposite
-  // signs and the totals telescope.
-  const unsigned* info;
   const D4* in;
   D4* out;
   const D4* avg;
@@ -355,7 +380,8 @@ struct UpdArgsM {
   int flags;
 };
 
-__global__ void __launch_bounds__(256) k_update_mixed(DMesh m, UpdArgsM a, DParams prm, DErr err) {
+__global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArgsM a, DParams prm,
+                                                      DErr err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
   if (i == 0) {
@@ -366,19 +392,19 @@ __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, UpdArgsM a, DPara
   if (i < m.N) {
     const D4 x = a.in[i];
     KSum sm{x.a, 0}, sE{x.b, 0}, sx{x.c, 0}, sy{x.d, 0};
-    const int kb = m.off[i], ke = m.off[i + 1];
-    for (int k0 = kb; k0 < ke; k0 += CH) {
+    const int deg = m.off[i + 1] - m.off[i], eb = ell_base(t, i);
+    for (int s0 = 0; s0 < deg; s0 += CH) {
       unsigned inf[CH];
       F4 fl[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
-        if (k0 + u < ke) inf[u] = a.info[k0 + u];
+        if (s0 + u < deg) inf[u] = t.einfo[eb + 64 * (s0 + u)];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
-        if (k0 + u < ke) fl[u] = a.flux[inf[u] & INFO_FACE];
+        if (s0 + u < deg) fl[u] = a.flux[inf[u] & INFO_FACE];
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
-        if (k0 + u >= ke) break;
+        if (s0 + u >= deg) break;
         const double s = (inf[u] & INFO_SECOND) ? dt : -dt;
         sm.add(s * static_cast<double>(fl[u].a));
         sE.add(s * static_cast<double>(fl[u].b));

@@ -91,7 +91,6 @@ struct rg_ctx {
   MTables mt{};
   F4 *primM = nullptr, *mid_primM = nullptr, *fluxM = nullptr;
   RecM* recM = nullptr;
-  unsigned* minfo = nullptr;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -148,7 +147,6 @@ void free_mesh(rg_ctx* c) {
   c->staging = c->partials = c->wbuf = nullptr;
   c->mt = MTables{};
   c->primM = c->mid_primM = c->fluxM = nullptr;
-  c->minfo = nullptr;
   c->recM = nullptr;
   c->has_mesh = c->has_cells = c->has_ext = c->has_flux = false;
 }
@@ -328,7 +326,6 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
       }));
       UpdArgsM a{};
       a.flux = c->fluxM;
-      a.info = c->minfo;
       a.dt_slot = c->words + cur;
       a.step = s;
       if (stage == 0) {
@@ -347,7 +344,7 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
         a.flags = UPD_AVERAGE | UPD_CFL;
       }
       HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
-        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c),
+        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, a, dparams(c),
                                                                 derr(c, s, 3 + 3 * stage));
       }));
     }
@@ -573,27 +570,20 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, dalloc(c, &c->partials, blocks_for(std::max(N, F))));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->prm.precision == RG_MIXED) {
-    FaceRecM* frecM;
-    FaceFrameM* fframeM;
-    CellGeoM* cgeoM;
-    HIPCHK(c, dalloc(c, &frecM, F));
-    HIPCHK(c, dalloc(c, &fframeM, F));
-    HIPCHK(c, dalloc(c, &cgeoM, N));
-    HIPCHK(c, dalloc(c, &c->primM, N));
-    HIPCHK(c, dalloc(c, &c->mid_primM, N));
-    HIPCHK(c, dalloc(c, &c->fluxM, F));
-    HIPCHK(c, dalloc(c, &c->recM, N));
     // Incidence words of the FP32 update.  Periodic copies of a boundary
     // face: the real cell r of face f sees ghost g (a translate of cell b by
     // shift q); the copy of f on b's side is the face between b and a ghost
-    // of r translated by -q.  Both cells read the lower-numbered copy.
+    // of r translated by -q.  Both cells read the lower-numbered copy, so
+    // they add one FP32 flux with opposite signs and the totals telescope.
     std::vector<unsigned> minfo(static_cast<size_t>(std::max(nnz, 1)));
     for (int k = 0; k < nnz; ++k) minfo[k] = inc[k].info & (INFO_FACE | INFO_SECOND);
     if (c->prm.bc == RG_BC_PERIODIC && G > 0) {
       double x0 = m->pt_x[0], x1 = x0, y0 = m->pt_y[0], y1 = y0;
       for (int i = 1; i < N; ++i) {
-        x0 = std::min(x0, m->pt_x[i]); x1 = std::max(x1, m->pt_x[i]);
-        y0 = std::min(y0, m->pt_y[i]); y1 = std::max(y1, m->pt_y[i]);
+        x0 = std::min(x0, m->pt_x[i]);
+        x1 = std::max(x1, m->pt_x[i]);
+        y0 = std::min(y0, m->pt_y[i]);
+        y1 = std::max(y1, m->pt_y[i]);
       }
       const double hw = 0.5 * (x1 - x0), hh = 0.5 * (y1 - y0);
       auto key = [](int r, int b, int qx, int qy) {
@@ -614,27 +604,62 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
         bface.emplace(key(r, o, qx, qy), f);
         twin_key[f] = key(o, r, -qx, -qy);
       }
-      for (int i = 0; i < N; ++i)
-        for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-          const int f = static_cast<int>(inc[k].info & INFO_FACE);
-          if (twin_key[f] == ~0ull) continue;
-          const auto it = bface.find(twin_key[f]);
-          if (it == bface.end() || it->second >= f) continue;
-          const int t = it->second;  // cell i is the ghost side of face t
-          minfo[k] = static_cast<unsigned>(t) | (m->face_second[t] >= N ? INFO_SECOND : 0u);
-        }
+      for (int k = 0; k < nnz; ++k) {
+        const int f = static_cast<int>(inc[k].info & INFO_FACE);
+        if (twin_key[f] == ~0ull) continue;
+        const auto it = bface.find(twin_key[f]);
+        if (it == bface.end() || it->second >= f) continue;
+        const int t = it->second;  // this cell is the ghost side of face t
+        minfo[k] = static_cast<unsigned>(t) | (m->face_second[t] >= N ? INFO_SECOND : 0u);
+      }
     }
-    unsigned* dinfo;
-    HIPCHK(c, upload(c, &dinfo, minfo.data(), minfo.size()));  // synchronised below
+    // wavefront-sliced ELL of the incidences
+    const int nw = (N + 63) / 64;
+    std::vector<int> wofs(static_cast<size_t>(nw) + 1, 0);
+    for (int w = 0; w < nw; ++w) {
+      int width = 0;
+      for (int i = 64 * w; i < std::min(N, 64 * w + 64); ++i)
+        width = std::max(width, m->csr_off[i + 1] - m->csr_off[i]);
+      wofs[w + 1] = wofs[w] + 64 * width;
+    }
+    const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
+    std::vector<int> enbr(ne, 0);
+    std::vector<unsigned> einfo(ne, 0u);
+    for (int i = 0; i < N; ++i) {
+      const int base = wofs[i / 64] + (i % 64);
+      for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
+        const size_t at = static_cast<size_t>(base) + 64 * static_cast<size_t>(k - m->csr_off[i]);
+        enbr[at] = inc[k].nbr;
+        einfo[at] = minfo[k];
+      }
+    }
+    FaceM* faceM;
+    CellGeoM* cgeoM;
+    int *dwofs, *denbr;
+    F4* degeo;
+    unsigned* deinfo;
+    HIPCHK(c, dalloc(c, &faceM, F));
+    HIPCHK(c, dalloc(c, &cgeoM, N));
+    HIPCHK(c, upload(c, &dwofs, wofs.data(), wofs.size()));
+    HIPCHK(c, upload(c, &denbr, enbr.data(), ne));
+    HIPCHK(c, upload(c, &deinfo, einfo.data(), ne));
+    HIPCHK(c, dalloc(c, &degeo, ne));
+    HIPCHK(c, hipMemsetAsync(degeo, 0, sizeof(F4) * ne, c->stream));
+    HIPCHK(c, dalloc(c, &c->primM, N));
+    HIPCHK(c, dalloc(c, &c->mid_primM, N));
+    HIPCHK(c, dalloc(c, &c->fluxM, F));
+    HIPCHK(c, dalloc(c, &c->recM, N));
     HIPCHK(c, launch(c, KID_GEO, [&] {
-      k_geo_mixed_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, frecM, fframeM);
+      k_geo_mixed_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, faceM);
     }));
     HIPCHK(c, launch(c, KID_GEO, [&] {
       k_geo_mixed_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeoM);
     }));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->mt = MTables{frecM, fframeM, cgeoM};
-    c->minfo = dinfo;
+    HIPCHK(c, launch(c, KID_GEO, [&] {
+      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, dwofs, degeo);
+    }));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above are uploaded
+    c->mt = MTables{faceM, cgeoM, dwofs, denbr, degeo, deinfo};
   }
   c->dm = d;
   c->N = N;
