@@ -87,7 +87,21 @@ struct DMesh {
   const FaceFrame* fframe;
   const double* area;
   const double* width;  // GetWidth = sqrt(GetVolume/pi)
+  // Incidences in wavefront-sliced ELL order: the cells of wavefront w are
+  // 64w..64w+63; slot s (the s-th face in ascending order) of lane l is entry
+  // wofs[w] + 64 s + l.  One slot load across a wavefront reads contiguous
+  // memory, where a CSR walk reads one scattered line per lane.
+  const int* wofs;
+  const int* enbr;       // neighbour id
+  const unsigned* einfo; // face | INFO_SECOND | INFO_MIRROR
+  const D2* eh;          // r_ij/2 as seen from the cell (negated on the second side)
+  const D2* eo;          // face centroid minus the cell's CM
+  const double* earea;   // face area
 };
+
+__device__ __forceinline__ int ell_base(const DMesh& m, int i) {
+  return m.wofs[i >> 6] + (i & 63);
+}
 
 struct DParams {
   double gamma, cfl, delta_v, theta, delta_p;
@@ -315,13 +329,35 @@ __device__ __forceinline__ void minmax(Cell& mx, Cell& mn, const Cell& t) {
 // dependent index -> data gathers); accumulation stays in face order.
 constexpr int CH = 4;
 
+// ELL copies of the per-incidence geometry (one thread per cell over its
+// faces): the same values the slope and update loops would read through the
+// face index, so the results do not change.
+__global__ void __launch_bounds__(256) k_geo_ell(DMesh m, D2* __restrict__ eh, D2* __restrict__ eo,
+                                                 double* __restrict__ earea) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const D2 ci = m.cm[i];
+  const int base = ell_base(m, i);
+  for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
+    const unsigned info = m.inc[k].info;
+    const int f = static_cast<int>(info & INFO_FACE);
+    const FaceRec r = m.frec[f];
+    const bool second = (info & INFO_SECOND) != 0;
+    const int at = base + 64 * (k - m.off[i]);
+    eh[at] = D2{second ? -r.hx : r.hx, second ? -r.hy : r.hy};
+    eo[at] = D2{r.cx - ci.x, r.cy - ci.y};
+    earea[at] = m.area[f];
+  }
+}
+
 // calc_slope (LinearGaussImproved.cpp:425-494) with the geometric half
 // taken from CellGeo / FaceRec.  Writes the cell's record.
 __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
                                                 Rec* __restrict__ out, DParams prm, DErr err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
-  const int b = m.off[i], e = m.off[i + 1];
+  const int b = m.off[i], e = m.off[i + 1], deg = e - b;
+  const int eb = ell_base(m, i);
   const Cell cell = ld_cell(q, i);
   const CellGeo g = m.cgeo[i];
   const int kind = static_cast<int>(g.kind);
@@ -364,26 +400,24 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ 
     // own term first (LinearGaussImproved.cpp:91-104).  A cell on this path
     // has no mirror face, so one set of neighbour extrema serves both limiters.
     Cell vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
-    for (int k0 = b; k0 < e; k0 += CH) {
-      Inc ic[CH];
-      FaceRec fr[CH];
+    for (int s0 = 0; s0 < deg; s0 += CH) {
+      int nb[CH];
+      D2 h[CH];
       Cell nc[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
-        if (k0 + u < e) ic[u] = m.inc[k0 + u];
-#pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (k0 + u < e) {
-          fr[u] = m.frec[ic[u].info & INFO_FACE];
-          nc[u] = fetch(m, q, ic[u].nbr);
+        if (s0 + u < deg) {
+          nb[u] = m.enbr[eb + 64 * (s0 + u)];
+          h[u] = m.eh[eb + 64 * (s0 + u)];
         }
 #pragma unroll
+      for (int u = 0; u < CH; ++u)
+        if (s0 + u < deg) nc[u] = fetch(m, q, nb[u]);
+#pragma unroll
       for (int u = 0; u < CH; ++u) {
-        if (k0 + u >= e) break;
-        const bool second = (ic[u].info & INFO_SECOND) != 0;
-        const double hx = second ? -fr[u].hx : fr[u].hx;
-        const double hy = second ? -fr[u].hy : fr[u].hy;
-        if (k0 + u == b) {
+        if (s0 + u >= deg) break;
+        const double hx = h[u].x, hy = h[u].y;
+        if (s0 + u == 0) {
           vcx = Cell{cell.d * hx, cell.p * hx, cell.vx * hx, cell.vy * hx};
           vcy = Cell{cell.d * hy, cell.p * hy, cell.vx * hy, cell.vy * hy};
         } else {
@@ -410,24 +444,18 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ 
     const bool shock = ((sx.vx + sy.vy) * g.width < (-prm.delta_v) * cs) || (pr < prm.delta_p);
     double psi0 = 1, psi1 = 1, psi2 = 1, psi3 = 1;
     if (!shock) {
-      // slope_limit (LinearGaussImproved.cpp:174-300)
+      // slope_limit (LinearGaussImproved.cpp:174-300); interp's centroid
+      // minus CM is the precomputed offset
       const Cell maxd = sub(bmax, cell), mind = sub(bmin, cell);
-      for (int k0 = b; k0 < e; k0 += CH) {
-        unsigned fi[CH];
-        D2 c2[CH];
+      for (int s0 = 0; s0 < deg; s0 += CH) {
+        D2 o[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u)
-          if (k0 + u < e) fi[u] = m.inc[k0 + u].info & INFO_FACE;
-#pragma unroll
-        for (int u = 0; u < CH; ++u)
-          if (k0 + u < e) {
-            const FaceRec fr = m.frec[fi[u]];
-            c2[u] = D2{fr.cx, fr.cy};
-          }
+          if (s0 + u < deg) o[u] = m.eo[eb + 64 * (s0 + u)];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-          if (k0 + u >= e) break;
-          const Cell cv = interp(cell, sx, sy, c2[u].x, c2[u].y, g.cmx, g.cmy);
+          if (s0 + u >= deg) break;
+          const Cell cv = interp_off(cell, sx, sy, o[u].x, o[u].y);
           const Cell dphi = sub(cv, cell);
           limit_field(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, psi0);
           limit_field(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, psi1);
@@ -438,11 +466,10 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ 
     } else {
       // shocked_slope_limit (LinearGaussImproved.cpp:302-399)
       const Cell maxd = sub(amax, cell), mind = sub(amin, cell);
-      for (int k = b; k < e; ++k) {
-        const Inc ic = m.inc[k];
-        const FaceRec fr = m.frec[ic.info & INFO_FACE];
-        const Cell nc = fetch(m, q, ic.nbr);
-        const Cell cv = interp(cell, sx, sy, fr.cx, fr.cy, g.cmx, g.cmy);
+      for (int sl = 0; sl < deg; ++sl) {
+        const D2 o = m.eo[eb + 64 * sl];
+        const Cell nc = fetch(m, q, m.enbr[eb + 64 * sl]);
+        const Cell cv = interp_off(cell, sx, sy, o.x, o.y);
         const Cell dphi = sub(cv, cell);
         shocked_field(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, prm.theta, psi0);
         shocked_field(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, prm.theta, psi1);
@@ -647,24 +674,23 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
       eE += dt * 0.0;
     }
     if (a.flags & UPD_FACES) {
-      const int kb = m.off[i], ke = m.off[i + 1];
-      for (int k0 = kb; k0 < ke; k0 += CH) {
+      const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
+      for (int s0 = 0; s0 < deg; s0 += CH) {
         unsigned inf[CH];
         D4 fl[CH];
         double ar[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u)
-          if (k0 + u < ke) inf[u] = m.inc[k0 + u].info;
-#pragma unroll
-        for (int u = 0; u < CH; ++u)
-          if (k0 + u < ke) {
-            const int f = static_cast<int>(inf[u] & INFO_FACE);
-            fl[u] = a.flux[f];
-            ar[u] = m.area[f];
+          if (s0 + u < deg) {
+            inf[u] = m.einfo[eb + 64 * (s0 + u)];
+            ar[u] = m.earea[eb + 64 * (s0 + u)];
           }
 #pragma unroll
+        for (int u = 0; u < CH; ++u)
+          if (s0 + u < deg) fl[u] = a.flux[inf[u] & INFO_FACE];
+#pragma unroll
         for (int u = 0; u < CH; ++u) {
-          if (k0 + u >= ke) break;
+          if (s0 + u >= deg) break;
           const double s = dt * ar[u];
           const double dm = fl[u].a * s, dE = fl[u].b * s, dx = fl[u].c * s, dy = fl[u].d * s;
           if (inf[u] & INFO_SECOND) {

@@ -80,6 +80,15 @@ __device__ __forceinline__ Cell interp(const Cell& c, const Cell& sx, const Cell
   return r;
 }
 
+// interp with the centroid-minus-CM offsets already formed
+__device__ __forceinline__ Cell interp_off(const Cell& c, const Cell& sx, const Cell& sy, double ox,
+                                           double oy) {
+  Cell r = c;
+  add_mult(r, sx, ox);
+  add_mult(r, sy, oy);
+  return r;
+}
+
 // One face: convert_to_primitive on both sides, rotate into the face frame,
 // Hllc, rotate_back (simple_flux_calculator.cpp:9-15,130-165; hllc.cpp:97-154).
 // The HLLC branch uses only one side's energy and tangential velocity, so

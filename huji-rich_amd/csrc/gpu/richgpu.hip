@@ -525,6 +525,41 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, upload(c, &gorig, G ? m->ghost_orig : &zero, G ? G : 1));
   HIPCHK(c, upload(c, &gsrc_d, gsrc.data(), gsrc.size()));
   HIPCHK(c, upload(c, &gn_d, gn.data(), gn.size()));
+  // wavefront-sliced ELL of the incidences (slot s of a cell = its s-th
+  // face in ascending order)
+  const int nw = (N + 63) / 64;
+  std::vector<int> wofs(static_cast<size_t>(nw) + 1, 0);
+  for (int w = 0; w < nw; ++w) {
+    int wd = 0;
+    for (int i = 64 * w; i < std::min(N, 64 * w + 64); ++i)
+      wd = std::max(wd, m->csr_off[i + 1] - m->csr_off[i]);
+    wofs[w + 1] = wofs[w] + 64 * wd;
+  }
+  const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
+  auto ell_at = [&](int i, int k) {
+    return static_cast<size_t>(wofs[i / 64] + i % 64) +
+           64 * static_cast<size_t>(k - m->csr_off[i]);
+  };
+  std::vector<int> enbr(ne, 0);
+  std::vector<unsigned> einfo(ne, 0u);
+  for (int i = 0; i < N; ++i)
+    for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
+      enbr[ell_at(i, k)] = inc[k].nbr;
+      einfo[ell_at(i, k)] = inc[k].info;
+    }
+  int *wofs_d, *enbr_d;
+  unsigned* einfo_d;
+  D2 *eh, *eo;
+  double* earea;
+  HIPCHK(c, upload(c, &wofs_d, wofs.data(), wofs.size()));
+  HIPCHK(c, upload(c, &enbr_d, enbr.data(), ne));
+  HIPCHK(c, upload(c, &einfo_d, einfo.data(), ne));
+  HIPCHK(c, dalloc(c, &eh, ne));
+  HIPCHK(c, dalloc(c, &eo, ne));
+  HIPCHK(c, dalloc(c, &earea, ne));
+  HIPCHK(c, hipMemsetAsync(eh, 0, sizeof(D2) * ne, c->stream));
+  HIPCHK(c, hipMemsetAsync(eo, 0, sizeof(D2) * ne, c->stream));
+  HIPCHK(c, hipMemsetAsync(earea, 0, sizeof(double) * ne, c->stream));
   HIPCHK(c, dalloc(c, &cgeo, N));
   HIPCHK(c, dalloc(c, &frec, F));
   HIPCHK(c, dalloc(c, &fframe, F));
@@ -552,12 +587,21 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   d.fframe = fframe;
   d.area = area;
   d.width = width;
+  d.wofs = wofs_d;
+  d.enbr = enbr_d;
+  d.einfo = einfo_d;
+  d.eh = eh;
+  d.eo = eo;
+  d.earea = earea;
   // static geometry tables
   HIPCHK(c, launch(c, KID_GEO, [&] {
     k_geo_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, frec, fframe, area);
   }));
   HIPCHK(c, launch(c, KID_GEO, [&] {
     k_geo_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeo, width);
+  }));
+  HIPCHK(c, launch(c, KID_GEO, [&] {
+    k_geo_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, eh, eo, earea);
   }));
   // state
   HIPCHK(c, dalloc(c, &c->prim, N));
@@ -613,36 +657,16 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
         minfo[k] = static_cast<unsigned>(t) | (m->face_second[t] >= N ? INFO_SECOND : 0u);
       }
     }
-    // wavefront-sliced ELL of the incidences
-    const int nw = (N + 63) / 64;
-    std::vector<int> wofs(static_cast<size_t>(nw) + 1, 0);
-    for (int w = 0; w < nw; ++w) {
-      int width = 0;
-      for (int i = 64 * w; i < std::min(N, 64 * w + 64); ++i)
-        width = std::max(width, m->csr_off[i + 1] - m->csr_off[i]);
-      wofs[w + 1] = wofs[w] + 64 * width;
-    }
-    const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
-    std::vector<int> enbr(ne, 0);
-    std::vector<unsigned> einfo(ne, 0u);
-    for (int i = 0; i < N; ++i) {
-      const int base = wofs[i / 64] + (i % 64);
-      for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-        const size_t at = static_cast<size_t>(base) + 64 * static_cast<size_t>(k - m->csr_off[i]);
-        enbr[at] = inc[k].nbr;
-        einfo[at] = minfo[k];
-      }
-    }
+    std::vector<unsigned> einfoM(ne, 0u);
+    for (int i = 0; i < N; ++i)
+      for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) einfoM[ell_at(i, k)] = minfo[k];
     FaceM* faceM;
     CellGeoM* cgeoM;
-    int *dwofs, *denbr;
     F4* degeo;
     unsigned* deinfo;
     HIPCHK(c, dalloc(c, &faceM, F));
     HIPCHK(c, dalloc(c, &cgeoM, N));
-    HIPCHK(c, upload(c, &dwofs, wofs.data(), wofs.size()));
-    HIPCHK(c, upload(c, &denbr, enbr.data(), ne));
-    HIPCHK(c, upload(c, &deinfo, einfo.data(), ne));
+    HIPCHK(c, upload(c, &deinfo, einfoM.data(), ne));
     HIPCHK(c, dalloc(c, &degeo, ne));
     HIPCHK(c, hipMemsetAsync(degeo, 0, sizeof(F4) * ne, c->stream));
     HIPCHK(c, dalloc(c, &c->primM, N));
@@ -656,10 +680,10 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       k_geo_mixed_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeoM);
     }));
     HIPCHK(c, launch(c, KID_GEO, [&] {
-      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, dwofs, degeo);
+      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, wofs_d, degeo);
     }));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above are uploaded
-    c->mt = MTables{faceM, cgeoM, dwofs, denbr, degeo, deinfo};
+    c->mt = MTables{faceM, cgeoM, wofs_d, enbr_d, degeo, deinfo};
   }
   c->dm = d;
   c->N = N;
