@@ -8,6 +8,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+TANGENT_PATH = os.path.join(_HERE, "build", "liboracle_tangent.so")
 
 CODES = {
     0: "ok",
@@ -85,6 +86,27 @@ def lib():
                              ctypes.c_void_p, ctypes.c_void_p, Quad_, Quad_, ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+_tlib = None
+
+
+def tlib():
+    """The forward-mode build of the same source (dual numbers)."""
+    global _tlib
+    if _tlib is None:
+        if not os.path.exists(TANGENT_PATH):
+            raise RuntimeError(f"{TANGENT_PATH} not built (make -C oracle)")
+        L = ctypes.CDLL(TANGENT_PATH)
+        P = ctypes.POINTER
+        L.rt_k.restype = ctypes.c_int
+        L.rt_run.restype = ctypes.c_int
+        L.rt_run.argtypes = [P(Mesh_), P(Params_), ctypes.c_int32, ctypes.c_int32,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                             P(ctypes.c_int32)]
+        _tlib = L
+    return _tlib
 
 
 def _p(a):
@@ -193,6 +215,40 @@ class Oracle:
                          _p(wx), _p(wy), _quad(c), _quad(x), _p(dts))
         self._check(e)
         return c[:, :self.N].copy(), x, dts[:nsteps]
+
+    def run_tangent(self, cells, dcells, nsteps, integrator=2, w=None):
+        """Forward-mode derivatives of init_extensives + run: cells (4, N)
+        initial state, dcells (K, 4, N) tangent directions of it (any K; run
+        in batches of the build's width).  Returns (cells, dcells, ext,
+        dext, dts) at the end: values as run() returns them (bit-identical)
+        and their directional derivatives (K, 4, N)."""
+        L = tlib()
+        kb = L.rt_k()
+        dcells = np.asarray(dcells, dtype=np.float64)
+        K = dcells.shape[0]
+        out_dc = np.zeros((K, 4, self.N))
+        out_dx = np.zeros((K, 4, self.N))
+        wx, wy = (None, None) if w is None else (np.ascontiguousarray(w[0]),
+                                                 np.ascontiguousarray(w[1]))
+        res = None
+        for k0 in range(0, max(K, 1), kb):
+            c = self._cells_ng(cells)
+            dc = np.zeros((kb, 4, self.NG))
+            nk = min(kb, K - k0)
+            dc[:nk, :, :self.N] = dcells[k0:k0 + nk]
+            x = np.zeros((4, self.N))
+            dx = np.zeros((kb, 4, self.N))
+            dts = np.zeros(max(nsteps, 1))
+            idx = ctypes.c_int32(-1)
+            rc = L.rt_run(ctypes.byref(self.m), ctypes.byref(self.prm), integrator, nsteps,
+                          _p(wx), _p(wy), _p(c), _p(dc), _p(x), _p(dx), _p(dts),
+                          ctypes.byref(idx))
+            if rc != 0:
+                raise OracleError(rc, idx.value)
+            out_dc[k0:k0 + nk] = dc[:nk, :, :self.N]
+            out_dx[k0:k0 + nk] = dx[:nk]
+            res = (c[:, :self.N].copy(), x, dts[:nsteps])
+        return res[0], out_dc, res[1], out_dx, res[2]
 
 
 def hllc(left, right, velocity):

@@ -17,40 +17,54 @@
 #define M_PI 3.14159265358979323846
 #endif
 
+/* State-dependent scalars are `real`: double here, a dual number when
+ * oracle/tangent.cpp compiles this file as C++ for forward-mode derivatives
+ * (RO_TANGENT).  Mesh geometry stays double in both builds. */
+#ifndef RO_TANGENT
+typedef double real;
+#define RO_VAL(x) (x)
+static double vabs_r(double x, double y) { return sqrt(x * x + y * y); }
+#endif
+
 #define RO_MAX_FACES 64
 
-typedef struct { double d, p, vx, vy; } cc_t;          /* ComputationalCell */
-typedef struct { double d, p, vx, vy, e, c; } prim_t;  /* Primitive         */
-typedef struct { double m, mx, my, E; } cons_t;        /* Conserved         */
+typedef struct { real d, p, vx, vy; } cc_t;          /* ComputationalCell */
+typedef struct { real d, p, vx, vy, e, c; } prim_t;  /* Primitive         */
+typedef struct { real m, mx, my, E; } cons_t;        /* Conserved         */
 
 /* misc/utils.cpp:3-8 */
-static int is_nan_ref(double x) {
+static int is_nan_ref(real x) {
   const int b1 = (x >= 0);
   const int b2 = (x < 0);
   return 1 != b1 + b2;
 }
 /* std::min / std::max: (b < a) ? b : a  and  (a < b) ? b : a */
-static double smin(double a, double b) { return (b < a) ? b : a; }
-static double smax(double a, double b) { return (a < b) ? b : a; }
+static real smin(real a, real b) { return (b < a) ? b : a; }
+static real smax(real a, real b) { return (a < b) ? b : a; }
 /* tessellation/geometry.cpp:18-21  abs(Vector2D) */
 static double vabs(double x, double y) { return sqrt(x * x + y * y); }
 
 /* ---------------------------------------------------------------- EOS -- */
 /* newtonian/common/ideal_gas.cpp:13-16 */
-double ro_dp2e(double g, double d, double p) { return p / d / (g - 1); }
+static real dp2e(double g, real d, real p) { return p / d / (g - 1); }
 /* ideal_gas.cpp:40-44 */
-double ro_dp2c(double g, double d, double p) { return sqrt(g * p / d); }
+static real dp2c(double g, real d, real p) { return sqrt(g * p / d); }
 /* ideal_gas.cpp:18-23 */
-int ro_de2p(double g, double d, double e, double* p) {
+static int de2p(double g, real d, real e, real* p) {
   if (e < 0) return RO_NEGATIVE_THERMAL;
   *p = (g - 1) * e * d;
   return RO_OK;
 }
+#ifndef RO_TANGENT
+double ro_dp2e(double g, double d, double p) { return dp2e(g, d, p); }
+double ro_dp2c(double g, double d, double p) { return dp2c(g, d, p); }
+int ro_de2p(double g, double d, double e, double* p) { return de2p(g, d, e, p); }
+#endif
 
 /* --------------------------------------------------------------- HLLC -- */
 /* hydrodynamic_variables.cpp:261-265; pow(abs(v),2) is abs(v)*abs(v) */
-static double total_energy_density(const prim_t* s) {
-  const double a = vabs(s->vx, s->vy);
+static real total_energy_density(const prim_t* s) {
+  const real a = vabs_r(s->vx, s->vy);
   return s->d * (0.5 * (a * a) + s->e);
 }
 /* hydrodynamic_variables.cpp:267-272 */
@@ -65,8 +79,8 @@ static cons_t primitive_to_conserved(const prim_t* s) {
 /* hydrodynamic_variables.cpp:281-290 with n = (1,0): n/abs(n) = (1,0)
  * exactly, ScalarProd(v, nn) = vx*1 + vy*0. */
 static cons_t primitive_to_flux_x(const prim_t* s) {
-  const double un = s->vx * 1.0 + s->vy * 0.0;
-  const double dun = s->d * un;
+  const real un = s->vx * 1.0 + s->vy * 0.0;
+  const real dun = s->d * un;
   cons_t r;
   r.m = s->d * un;
   r.mx = 1.0 * s->p + s->vx * dun;
@@ -75,20 +89,20 @@ static cons_t primitive_to_flux_x(const prim_t* s) {
   return r;
 }
 /* hllc.cpp:27-43 */
-static void estimate_wave_speeds(const prim_t* l, const prim_t* r, double* sl, double* ss,
-                                 double* sr) {
-  const double dl = l->d, pl = l->p, vl = l->vx, cl = l->c;
-  const double dr = r->d, pr = r->p, vr = r->vx, cr = r->c;
+static void estimate_wave_speeds(const prim_t* l, const prim_t* r, real* sl, real* ss,
+                                 real* sr) {
+  const real dl = l->d, pl = l->p, vl = l->vx, cl = l->c;
+  const real dr = r->d, pr = r->p, vr = r->vx, cr = r->c;
   *sl = smin(vl - cl, vr - cr);
   *sr = smax(vl + cl, vr + cr);
   *ss = (pr - pl + dl * vl * (*sl - vl) - dr * vr * (*sr - vr)) /
         (dl * (*sl - vl) - dr * (*sr - vr));
 }
 /* hllc.cpp:76-92 */
-static cons_t starred_state(const prim_t* s, double sk, double ss) {
-  const double dk = s->d, pk = s->p, uk = s->vx, vk = s->vy;
-  const double ds = dk * (sk - uk) / (sk - ss);
-  const double ek = total_energy_density(s);
+static cons_t starred_state(const prim_t* s, real sk, real ss) {
+  const real dk = s->d, pk = s->p, uk = s->vx, vk = s->vy;
+  const real ds = dk * (sk - uk) / (sk - ss);
+  const real ek = total_energy_density(s);
   cons_t r;
   r.m = ds;
   r.mx = ds * ss;
@@ -97,7 +111,7 @@ static cons_t starred_state(const prim_t* s, double sk, double ss) {
   return r;
 }
 /* f + s*(us - u), hydrodynamic_variables.cpp:217-236 */
-static cons_t add_scaled_diff(const cons_t* f, double s, const cons_t* us, const cons_t* u) {
+static cons_t add_scaled_diff(const cons_t* f, real s, const cons_t* us, const cons_t* u) {
   cons_t r;
   r.m = f->m + s * (us->m - u->m);
   r.mx = f->mx + (us->mx - u->mx) * s;
@@ -114,7 +128,7 @@ static int hllc(const prim_t* left, const prim_t* right, double velocity, cons_t
   l.vy = l.vy - 0.0 * velocity;
   r.vx = r.vx - 1.0 * velocity;
   r.vy = r.vy - 0.0 * velocity;
-  double sl, ss, sr;
+  real sl, ss, sr;
   estimate_wave_speeds(&l, &r, &sl, &ss, &sr);
   const cons_t ul = primitive_to_conserved(&l);
   const cons_t ur = primitive_to_conserved(&r);
@@ -135,13 +149,14 @@ static int hllc(const prim_t* left, const prim_t* right, double velocity, cons_t
     return RO_INVALID_WAVE_SPEEDS;
   /* back to the lab frame */
   f.E = f.E + (f.mx * (1.0 * velocity) + f.my * (0.0 * velocity) + 0.5 * f.m * velocity * velocity);
-  const double vm = velocity * f.m;
+  const real vm = velocity * f.m;
   f.mx = f.mx + 1.0 * vm;
   f.my = f.my + 0.0 * vm;
   *out = f;
   return RO_OK;
 }
 
+#ifndef RO_TANGENT
 int ro_hllc(const double L[6], const double R[6], double velocity, double out[4]) {
   const prim_t l = {L[0], L[1], L[2], L[3], L[4], L[5]};
   const prim_t r = {R[0], R[1], R[2], R[3], R[4], R[5]};
@@ -155,6 +170,7 @@ int ro_hllc(const double L[6], const double R[6], double velocity, double out[4]
   }
   return rc;
 }
+#endif
 
 /* simple_flux_calculator.cpp:9-15 */
 static prim_t convert_to_primitive(const cc_t* c, double g) {
@@ -163,8 +179,8 @@ static prim_t convert_to_primitive(const cc_t* c, double g) {
   r.p = c->p;
   r.vx = c->vx;
   r.vy = c->vy;
-  r.e = ro_dp2e(g, c->d, c->p);
-  r.c = ro_dp2c(g, c->d, c->p);
+  r.e = dp2e(g, c->d, c->p);
+  r.c = dp2c(g, c->d, c->p);
   return r;
 }
 /* simple_flux_calculator.cpp:130-140; Projection = ScalarProd/abs
@@ -245,7 +261,7 @@ void ro_ghost_cells(const ro_mesh* m, const ro_params* prm, ro_cells c) {
       const double an = vabs(nx, ny);
       nx = nx / an;
       ny = ny / an;
-      const double s2 = 2 * (t.vx * nx + t.vy * ny);
+      const real s2 = 2 * (t.vx * nx + t.vy * ny);
       t.vx = t.vx - nx * s2;
       t.vy = t.vy - ny * s2;
       put_cell(c, g, t);
@@ -376,9 +392,9 @@ static void get_boundary_slope(const ro_mesh* m, int32_t ci, const cc_t* cell, i
 }
 
 /* LinearGaussImproved.cpp:136-148 */
-static double pressure_ratio(const cc_t* cell, int n, const cc_t* ncell) {
-  double res = 1;
-  const double p = cell->p;
+static real pressure_ratio(const cc_t* cell, int n, const cc_t* ncell) {
+  real res = 1;
+  const real p = cell->p;
   for (int i = 0; i < n; ++i) {
     if (p > ncell[i].p)
       res = smin(res, ncell[i].p / p);
@@ -390,11 +406,11 @@ static double pressure_ratio(const cc_t* cell, int n, const cc_t* ncell) {
 
 /* one field of slope_limit (LinearGaussImproved.cpp:232-267); absguard
  * selects the velocity form of the +-1e-9 guard */
-static void limit_field(double dphi, double cval, double cell, double maxd, double mind,
-                        int absguard, double* psi) {
+static void limit_field(real dphi, real cval, real cell, real maxd, real mind,
+                        int absguard, real* psi) {
   if (fabs(dphi) > 0.05 * smax(fabs(maxd), fabs(mind)) || cval * cell < 0) {
-    const double hi = absguard ? fabs(1e-9 * cell) : 1e-9 * cell;
-    const double lo = absguard ? -fabs(1e-9 * cell) : -1e-9 * cell;
+    const real hi = absguard ? fabs(1e-9 * cell) : 1e-9 * cell;
+    const real lo = absguard ? -fabs(1e-9 * cell) : -1e-9 * cell;
     if (dphi > hi)
       *psi = smin(*psi, maxd / dphi);
     else if (dphi < lo)
@@ -424,7 +440,7 @@ static void slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, int n,
   const cc_t maxd = {cmax.d - cell->d, cmax.p - cell->p, cmax.vx - cell->vx, cmax.vy - cell->vy};
   const cc_t mind = {cmin.d - cell->d, cmin.p - cell->p, cmin.vx - cell->vx, cmin.vy - cell->vy};
   const double cmx = m->cm_x[ci], cmy = m->cm_y[ci];
-  double psi[4] = {1, 1, 1, 1};
+  real psi[4] = {1, 1, 1, 1};
   for (int i = 0; i < n; ++i) {
     double fx, fy;
     face_centroid(m, faces[i], &fx, &fy);
@@ -447,8 +463,8 @@ static void slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, int n,
 
 /* one field of shocked_slope_limit (LinearGaussImproved.cpp:343-366); the
  * guard has no abs() for any field (:346,352,358,364) */
-static void shocked_field(double dphi, double cval, double cell, double nval, double maxd,
-                          double mind, double theta, double* psi) {
+static void shocked_field(real dphi, real cval, real cell, real nval, real maxd,
+                          real mind, double theta, real* psi) {
   if (fabs(dphi) > 0.1 * smax(fabs(maxd), fabs(mind)) || cval * cell < 0) {
     if (fabs(dphi) > 1e-9 * cell) *psi = smin(*psi, smax(theta * (nval - cell) / dphi, 0.0));
   }
@@ -473,7 +489,7 @@ static void shocked_slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, 
   const cc_t maxd = {cmax.d - cell->d, cmax.p - cell->p, cmax.vx - cell->vx, cmax.vy - cell->vy};
   const cc_t mind = {cmin.d - cell->d, cmin.p - cell->p, cmin.vx - cell->vx, cmin.vy - cell->vy};
   const double cmx = m->cm_x[ci], cmy = m->cm_y[ci];
-  double psi[4] = {1, 1, 1, 1};
+  real psi[4] = {1, 1, 1, 1};
   for (int i = 0; i < n; ++i) {
     double fx, fy;
     face_centroid(m, faces[i], &fx, &fy);
@@ -526,7 +542,7 @@ static int calc_slope(const ro_mesh* m, const ro_params* prm, ro_cells cng, int3
   }
   if (naive) *naive = *res;
   if (prm->slf) {
-    const double cs = ro_dp2c(prm->gamma, cell.d, cell.p);
+    const real cs = dp2c(prm->gamma, cell.d, cell.p);
     const double width = sqrt(m->tess_volume[ci] / M_PI);
     /* is_shock, LinearGaussImproved.cpp:150-157 */
     const int cond1 = (res->x.vx + res->y.vy) * width < (-prm->delta_v) * cs;
@@ -539,7 +555,7 @@ static int calc_slope(const ro_mesh* m, const ro_params* prm, ro_cells cng, int3
   return RO_OK;
 }
 
-static void put_slope(double* const s[8], int32_t i, const slope_t* v) {
+static void put_slope(real* const s[8], int32_t i, const slope_t* v) {
   s[0][i] = v->x.d;
   s[1][i] = v->x.p;
   s[2][i] = v->x.vx;
@@ -549,7 +565,7 @@ static void put_slope(double* const s[8], int32_t i, const slope_t* v) {
   s[6][i] = v->y.vx;
   s[7][i] = v->y.vy;
 }
-static slope_t get_slope(double* const s[8], int32_t i) {
+static slope_t get_slope(real* const s[8], int32_t i) {
   slope_t v;
   v.x.d = s[0][i];
   v.x.p = s[1][i];
@@ -564,7 +580,7 @@ static slope_t get_slope(double* const s[8], int32_t i) {
 
 /* ghost gradient: RigidWallGenerator.cpp:39-54 (zero) or
  * PeriodicGhostGenerator.cpp:19-24 (slope of the original cell) */
-static slope_t ghost_gradient(const ro_mesh* m, const ro_params* prm, double* const slopes[8],
+static slope_t ghost_gradient(const ro_mesh* m, const ro_params* prm, real* const slopes[8],
                               int32_t g) {
   slope_t z;
   memset(&z, 0, sizeof(z));
@@ -581,8 +597,8 @@ static ro_error err_make(int code, int32_t index) {
 
 /* LinearGaussImproved::operator() (LinearGaussImproved.cpp:536-635).
  * cng has N+G entries with the ghost slots filled by ro_ghost_cells. */
-ro_error ro_lgi(const ro_mesh* m, const ro_params* prm, ro_cells cng, double* const slopes[8],
-                double* const naive[8], ro_cells left, ro_cells right) {
+ro_error ro_lgi(const ro_mesh* m, const ro_params* prm, ro_cells cng, real* const slopes[8],
+                real* const naive[8], ro_cells left, ro_cells right) {
   const int32_t N = m->n_cells;
   const int nt = prm->nthreads > 1 ? prm->nthreads : 1;
   ro_error err = err_make(RO_OK, -1);
@@ -686,7 +702,7 @@ static int modular_face_flux(const ro_mesh* m, const ro_params* prm, int32_t f, 
  * (simple_flux_calculator.cpp:17-25) */
 static prim_t reflect_prim(const prim_t* s, double ax, double ay) {
   prim_t r = *s;
-  const double s2 = 2 * (s->vx * ax + s->vy * ay);
+  const real s2 = 2 * (s->vx * ax + s->vy * ay);
   const double a = vabs(ax, ay);
   const double a2 = a * a;
   r.vx = ax * s2 / a2 - s->vx;
@@ -739,14 +755,14 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
   const int nt = prm->nthreads > 1 ? prm->nthreads : 1;
   ro_ghost_cells(m, prm, cng);
   ro_cells left = {0, 0, 0, 0}, right = {0, 0, 0, 0};
-  double* sbuf = NULL;
-  double* lbuf = NULL;
+  real* sbuf = NULL;
+  real* lbuf = NULL;
   (void)N;
   (void)NG;
   if (prm->order == 2) {
-    sbuf = (double*)malloc(sizeof(double) * 8 * (size_t)N);
-    lbuf = (double*)malloc(sizeof(double) * 8 * (size_t)F);
-    double* slopes[8];
+    sbuf = (real*)malloc(sizeof(real) * 8 * (size_t)N);
+    lbuf = (real*)malloc(sizeof(real) * 8 * (size_t)F);
+    real* slopes[8];
     for (int k = 0; k < 8; ++k) slopes[k] = sbuf + (size_t)k * N;
     left.d = lbuf;
     left.p = lbuf + F;
@@ -803,16 +819,16 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
  * (physical_geometry.cpp:8-11).  The OpenMP form gathers each cell's
  * faces in ascending order, which applies the same operations in the same
  * order to every cell. */
-void ro_update_extensives(const ro_mesh* m, const ro_params* prm, ro_ext flux, double dt,
+void ro_update_extensives(const ro_mesh* m, const ro_params* prm, ro_ext flux, real dt,
                           ro_ext ext) {
   const int32_t N = m->n_cells;
   const int nt = prm->nthreads > 1 ? prm->nthreads : 1;
   if (nt == 1) {
     for (int32_t f = 0; f < m->n_faces; ++f) {
       const double area = vabs(m->v1x[f] - m->v2x[f], m->v1y[f] - m->v2y[f]);
-      const double s = dt * area;
-      const double dm = flux.m[f] * s, dE = flux.E[f] * s, dx = flux.Mx[f] * s,
-                   dy = flux.My[f] * s;
+      const real s = dt * area;
+      const real dm = flux.m[f] * s, dE = flux.E[f] * s, dx = flux.Mx[f] * s,
+                 dy = flux.My[f] * s;
       const int32_t a = m->face_first[f], b = m->face_second[f];
       if (a >= 0 && a < N) {
         ext.m[a] -= dm;
@@ -831,13 +847,13 @@ void ro_update_extensives(const ro_mesh* m, const ro_params* prm, ro_ext flux, d
   }
 #pragma omp parallel for num_threads(nt) schedule(static)
   for (int32_t c = 0; c < N; ++c) {
-    double em = ext.m[c], eE = ext.E[c], ex = ext.Mx[c], ey = ext.My[c];
+    real em = ext.m[c], eE = ext.E[c], ex = ext.Mx[c], ey = ext.My[c];
     for (int32_t k = m->csr_off[c]; k < m->csr_off[c + 1]; ++k) {
       const int32_t f = m->csr_face[k];
       const double area = vabs(m->v1x[f] - m->v2x[f], m->v1y[f] - m->v2y[f]);
-      const double s = dt * area;
-      const double dm = flux.m[f] * s, dE = flux.E[f] * s, dx = flux.Mx[f] * s,
-                   dy = flux.My[f] * s;
+      const real s = dt * area;
+      const real dm = flux.m[f] * s, dE = flux.E[f] * s, dx = flux.Mx[f] * s,
+                 dy = flux.My[f] * s;
       if (m->face_first[f] == c) {
         em -= dm;
         eE -= dE;
@@ -861,13 +877,13 @@ void ro_update_extensives(const ro_mesh* m, const ro_params* prm, ro_ext flux, d
 /* regular_update (simple_cell_updater.cpp:22-59) */
 static int cell_update_one(const ro_mesh* m, double g, ro_ext ext, ro_cells cells, int32_t i) {
   const double volume = m->volume[i];
-  const double mass = ext.m[i];
-  const double d = mass / volume;
+  const real mass = ext.m[i];
+  const real d = mass / volume;
   if (d < 0) return RO_NEGATIVE_DENSITY;
-  const double vx = ext.Mx[i] / mass, vy = ext.My[i] / mass;
-  const double energy = ext.E[i] / mass - 0.5 * (vx * vx + vy * vy);
-  double p;
-  const int rc = ro_de2p(g, d, energy, &p);
+  const real vx = ext.Mx[i] / mass, vy = ext.My[i] / mass;
+  const real energy = ext.E[i] / mass - 0.5 * (vx * vx + vy * vy);
+  real p;
+  const int rc = de2p(g, d, energy, &p);
   if (rc != RO_OK) return rc;
   cells.d[i] = d;
   cells.p[i] = p;
@@ -906,33 +922,33 @@ ro_error ro_update_cells(const ro_mesh* m, const ro_params* prm, ro_ext ext, ro_
 
 /* TimeStepCalculator::operator[] (simple_cfl.cpp:28-47); GetWidth is
  * sqrt(GetVolume/pi) (VoronoiMesh.cpp:410-413) */
-static double cfl_cell(const ro_mesh* m, double g, ro_cells cells, const double* wx,
-                       const double* wy, int32_t i) {
-  double res = 0;
+static real cfl_cell(const ro_mesh* m, double g, ro_cells cells, const double* wx,
+                     const double* wy, int32_t i) {
+  real res = 0;
   const double radius = sqrt(m->tess_volume[i] / M_PI);
-  const double c = ro_dp2c(g, cells.d[i], cells.p[i]);
-  const double vx = cells.vx[i], vy = cells.vy[i];
+  const real c = dp2c(g, cells.d[i], cells.p[i]);
+  const real vx = cells.vx[i], vy = cells.vy[i];
   for (int32_t k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
     const int32_t f = m->csr_face[k];
     const double ex = wx ? wx[f] : 0.0, ey = wy ? wy[f] : 0.0;
-    res = fmax(res, (c + vabs(vx - ex, vy - ey)) / radius);
+    res = fmax(res, (c + vabs_r(vx - ex, vy - ey)) / radius);
   }
   return 1.0 / res;
 }
 
 /* SimpleCFL::operator() (simple_cfl.cpp:58-71) with lazy_min
  * (misc/lazy_list.hpp:185-191) */
-double ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells, const double* wx,
-              const double* wy) {
+real ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells, const double* wx,
+            const double* wy) {
   const int32_t N = m->n_cells;
   const int nt = prm->nthreads > 1 ? prm->nthreads : 1;
-  double res = cfl_cell(m, prm->gamma, cells, wx, wy, 0);
+  real res = cfl_cell(m, prm->gamma, cells, wx, wy, 0);
   if (nt == 1) {
     for (int32_t i = 1; i < N; ++i) res = smin(res, cfl_cell(m, prm->gamma, cells, wx, wy, i));
   } else {
 #pragma omp parallel num_threads(nt)
     {
-      double loc = res;
+      real loc = res;
 #pragma omp for schedule(static)
       for (int32_t i = 1; i < N; ++i) loc = smin(loc, cfl_cell(m, prm->gamma, cells, wx, wy, i));
 #pragma omp critical
@@ -946,10 +962,10 @@ double ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells, const doub
 void ro_init_extensives(const ro_mesh* m, const ro_params* prm, ro_cells cells, ro_ext ext) {
   for (int32_t i = 0; i < m->n_cells; ++i) {
     const double volume = m->volume[i];
-    const double mass = volume * cells.d[i];
-    const double vx = cells.vx[i], vy = cells.vy[i];
+    const real mass = volume * cells.d[i];
+    const real vx = cells.vx[i], vy = cells.vy[i];
     ext.m[i] = mass;
-    ext.E[i] = ro_dp2e(prm->gamma, cells.d[i], cells.p[i]) * mass +
+    ext.E[i] = dp2e(prm->gamma, cells.d[i], cells.p[i]) * mass +
                0.5 * mass * (vx * vx + vy * vy);
     ext.Mx[i] = vx * mass;
     ext.My[i] = vy * mass;
@@ -958,8 +974,8 @@ void ro_init_extensives(const ro_mesh* m, const ro_params* prm, ro_cells cells, 
 
 /* ExternalForceContribution with ZeroForce (hydrodynamics_2d.cpp:348-372,
  * zero_force.cpp:3-18): adds dt*0 to every extensive */
-static void zero_force(int32_t N, double dt, ro_ext ext) {
-  const double z = dt * 0.0;
+static void zero_force(int32_t N, real dt, ro_ext ext) {
+  const real z = dt * 0.0;
   for (int32_t i = 0; i < N; ++i) {
     ext.m[i] += z;
     ext.Mx[i] += 0.0 * dt;
@@ -968,7 +984,7 @@ static void zero_force(int32_t N, double dt, ro_ext ext) {
   }
 }
 
-static double* alloc_d(size_t n) { return (double*)malloc(sizeof(double) * (n ? n : 1)); }
+static real* alloc_d(size_t n) { return (real*)malloc(sizeof(real) * (n ? n : 1)); }
 
 /* hdsim::TimeAdvance (hdsim2d.cpp:149-241) and hdsim::TimeAdvance2Heun
  * (hdsim2d.cpp:311-449) on a static mesh (no rebuild, no Hilbert
@@ -976,16 +992,16 @@ static double* alloc_d(size_t n) { return (double*)malloc(sizeof(double) * (n ? 
 ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int32_t nsteps,
                 const double* wx, const double* wy, ro_cells cng, ro_ext ext, double* dt_out) {
   const int32_t N = m->n_cells, F = m->n_faces, NG = m->n_cells + m->n_ghosts;
-  double* fb = alloc_d(4 * (size_t)F);
+  real* fb = alloc_d(4 * (size_t)F);
   ro_ext flux = {fb, fb + F, fb + 2 * (size_t)F, fb + 3 * (size_t)F};
-  double* mb = alloc_d(4 * (size_t)N);
+  real* mb = alloc_d(4 * (size_t)N);
   ro_ext mid = {mb, mb + N, mb + 2 * (size_t)N, mb + 3 * (size_t)N};
-  double* cb = alloc_d(4 * (size_t)NG);
+  real* cb = alloc_d(4 * (size_t)NG);
   ro_cells midc = {cb, cb + NG, cb + 2 * (size_t)NG, cb + 3 * (size_t)NG};
   ro_error err = err_make(RO_OK, -1);
   for (int32_t step = 0; step < nsteps && err.code == RO_OK; ++step) {
-    const double dt = ro_cfl(m, prm, cng, wx, wy);
-    if (dt_out) dt_out[step] = dt;
+    const real dt = ro_cfl(m, prm, cng, wx, wy);
+    if (dt_out) dt_out[step] = RO_VAL(dt);
     if (integrator == 1) {
       err = ro_flux(m, prm, cng, wx, wy, flux);
       if (err.code != RO_OK) break;
@@ -995,10 +1011,10 @@ ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int3
     } else {
       err = ro_flux(m, prm, cng, wx, wy, flux);
       if (err.code != RO_OK) break;
-      memcpy(mid.m, ext.m, sizeof(double) * (size_t)N);
-      memcpy(mid.E, ext.E, sizeof(double) * (size_t)N);
-      memcpy(mid.Mx, ext.Mx, sizeof(double) * (size_t)N);
-      memcpy(mid.My, ext.My, sizeof(double) * (size_t)N);
+      memcpy(mid.m, ext.m, sizeof(real) * (size_t)N);
+      memcpy(mid.E, ext.E, sizeof(real) * (size_t)N);
+      memcpy(mid.Mx, ext.Mx, sizeof(real) * (size_t)N);
+      memcpy(mid.My, ext.My, sizeof(real) * (size_t)N);
       zero_force(N, dt, mid);
       ro_update_extensives(m, prm, flux, dt, mid);
       err = ro_update_cells(m, prm, mid, midc);

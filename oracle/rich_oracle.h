@@ -20,8 +20,16 @@
 
 #include <stdint.h>
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(RO_TANGENT)
 extern "C" {
+#endif
+
+/* scalar type of the state arrays: double, or the dual numbers of the
+ * forward-mode build (oracle/tangent.cpp) */
+#ifndef RO_TANGENT
+#define RO_REAL double
+#else
+#define RO_REAL real
 #endif
 
 typedef struct ro_mesh {
@@ -47,8 +55,8 @@ typedef struct ro_params {
 } ro_params;
 
 /* SoA views; arrays are read or written as the function states. */
-typedef struct ro_cells { double *d, *p, *vx, *vy; } ro_cells;
-typedef struct ro_ext { double *m, *E, *Mx, *My; } ro_ext;
+typedef struct ro_cells { RO_REAL *d, *p, *vx, *vy; } ro_cells;
+typedef struct ro_ext { RO_REAL *m, *E, *Mx, *My; } ro_ext;
 
 /* Error codes mirror the UniversalError messages of the reference. */
 enum {
@@ -62,6 +70,7 @@ enum {
 };
 typedef struct ro_error { int32_t code; int32_t index; } ro_error;
 
+#ifndef RO_TANGENT
 /* --- single-face physics (for known-answer tests) --- */
 /* prim = {density, pressure, vx, vy, energy, sound speed} (Primitive) */
 int ro_hllc(const double left[6], const double right[6], double velocity,
@@ -101,8 +110,9 @@ void ro_init_extensives(const ro_mesh* m, const ro_params* prm, ro_cells cells,
 ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator,
                 int32_t nsteps, const double* wx, const double* wy,
                 ro_cells cells_ng, ro_ext ext, double* dt_out);
+#endif /* RO_TANGENT */
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(RO_TANGENT)
 }
 #endif
 
