@@ -32,30 +32,30 @@ enum : int {
 };
 
 // std::min / std::max as the reference calls them
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 // abs(Vector2D), geometry.cpp:18-21
-__device__ __forceinline__ double vabs(double x, double y) { return sqrt(x * x + y * y); }
+__host__ __device__ __forceinline__ double vabs(double x, double y) { return sqrt(x * x + y * y); }
 // is_nan, misc/utils.cpp:3-8
-__device__ __forceinline__ bool is_nan_ref(double x) {
+__host__ __device__ __forceinline__ bool is_nan_ref(double x) {
   const int b1 = (x >= 0);
   const int b2 = (x < 0);
   return 1 != b1 + b2;
 }
 
 // IdealGas::dp2e / dp2c (ideal_gas.cpp:13-16, 40-44)
-__device__ __forceinline__ double dp2e(double g, double d, double p) { return p / d / (g - 1); }
-__device__ __forceinline__ double dp2c(double g, double d, double p) { return sqrt(g * p / d); }
+__host__ __device__ __forceinline__ double dp2e(double g, double d, double p) { return p / d / (g - 1); }
+__host__ __device__ __forceinline__ double dp2c(double g, double d, double p) { return sqrt(g * p / d); }
 
 // TotalEnergyDensity (hydrodynamic_variables.cpp:261-265): pow(abs(v),2) is
 // evaluated as abs(v)*abs(v)
-__device__ __forceinline__ double ted(const Prim& s) {
+__host__ __device__ __forceinline__ double ted(const Prim& s) {
   const double a = vabs(s.vx, s.vy);
   return s.d * (0.5 * (a * a) + s.e);
 }
 
 // rotate_back (simple_flux_calculator.cpp:142-151)
-__device__ __forceinline__ Cons rotate_back(const Cons& c, double nx, double ny, double an,
+__host__ __device__ __forceinline__ Cons rotate_back(const Cons& c, double nx, double ny, double an,
                                             double px, double py, double ap) {
   Cons r = c;
   r.mx = nx * c.mx / an + px * c.my / ap;
@@ -64,7 +64,7 @@ __device__ __forceinline__ Cons rotate_back(const Cons& c, double nx, double ny,
 }
 
 // ComputationalCellAddMult (computational_cell_2d.cpp:59-68)
-__device__ __forceinline__ void add_mult(Cell& r, const Cell& o, double s) {
+__host__ __device__ __forceinline__ void add_mult(Cell& r, const Cell& o, double s) {
   r.d += o.d * s;
   r.p += o.p * s;
   r.vx += o.vx * s;
@@ -72,7 +72,7 @@ __device__ __forceinline__ void add_mult(Cell& r, const Cell& o, double s) {
 }
 
 // interp / interp2 (LinearGaussImproved.cpp:159-172)
-__device__ __forceinline__ Cell interp(const Cell& c, const Cell& sx, const Cell& sy, double tx,
+__host__ __device__ __forceinline__ Cell interp(const Cell& c, const Cell& sx, const Cell& sy, double tx,
                                        double ty, double cmx, double cmy) {
   Cell r = c;
   add_mult(r, sx, tx - cmx);
@@ -81,7 +81,7 @@ __device__ __forceinline__ Cell interp(const Cell& c, const Cell& sx, const Cell
 }
 
 // interp with the centroid-minus-CM offsets already formed
-__device__ __forceinline__ Cell interp_off(const Cell& c, const Cell& sx, const Cell& sy, double ox,
+__host__ __device__ __forceinline__ Cell interp_off(const Cell& c, const Cell& sx, const Cell& sy, double ox,
                                            double oy) {
   Cell r = c;
   add_mult(r, sx, ox);
@@ -94,7 +94,7 @@ __device__ __forceinline__ Cell interp_off(const Cell& c, const Cell& sx, const 
 // The HLLC branch uses only one side's energy and tangential velocity, so
 // those two quantities (three IEEE divisions) are computed for that side
 // only; every value that is computed is evaluated exactly as above.
-__device__ __forceinline__ int hllc_face(const Cell& L, const Cell& R, double g, double nx,
+__host__ __device__ __forceinline__ int hllc_face(const Cell& L, const Cell& R, double g, double nx,
                                          double ny, double an, double px, double py, double ap,
                                          double velocity, Cons& out) {
   const double cl = dp2c(g, L.d, L.p), cr = dp2c(g, R.d, R.p);
