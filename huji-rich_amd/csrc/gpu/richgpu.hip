@@ -13,6 +13,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "kernels_mixed.hpp"
 
 using namespace rg;
@@ -34,13 +35,17 @@ enum KernelId : int {
   KID_SLOPES_MIXED,
   KID_FLUX_MIXED,
   KID_UPDATE_MIXED,
+  KID_ADJ_FLUX,
+  KID_ADJ_SLOPES,
+  KID_ADJ_CELL,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
                                              "k_flux1",       "k_update",   "k_face_states",
                                              "k_init_ext",    "k_fill_u64", "k_geo",
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
-                                             "k_flux_mixed",  "k_update_mixed"};
+                                             "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
+                                             "k_adj_slopes",  "k_adj_cell"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -70,6 +75,7 @@ struct rg_ctx {
   bool own_stream = false;
 
   int N = 0, F = 0, G = 0, nnz = 0;
+  int dm_ne = 0;  // ELL entries
   bool has_mesh = false, has_cells = false, has_ext = false, has_flux = false;
 
   std::vector<void*> mesh_bufs;  // freed on re-mesh
@@ -91,6 +97,19 @@ struct rg_ctx {
   MTables mt{};
   F4 *primM = nullptr, *mid_primM = nullptr, *fluxM = nullptr;
   RecM* recM = nullptr;
+
+  // adjoint: tables, the recorded forward run and the reverse buffers
+  bool ext_init = false;  // extensives are init_extensives(cells)
+  AdjMesh am{};
+  int adj_steps = 0, adj_every = 0, adj_nck = 0, adj_cap_every = 0, adj_cap_nck = 0;
+  std::vector<void*> adj_bufs;  // checkpoints and tape (sized per run)
+  D4 *ckP = nullptr, *ckX = nullptr, *finP = nullptr, *finX = nullptr;
+  D4 *tP = nullptr, *tX = nullptr, *tPm = nullptr, *tXm = nullptr;
+  double* tdt = nullptr;
+  D4 *gP = nullptr, *gX = nullptr, *ga = nullptr, *gb = nullptr, *gps = nullptr;
+  D4 *gslope = nullptr, *lb = nullptr, *rb = nullptr, *emit = nullptr;
+  double *dtb = nullptr, *vmin = nullptr;
+  int* imin = nullptr;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -147,6 +166,16 @@ void free_mesh(rg_ctx* c) {
   c->staging = c->partials = c->wbuf = nullptr;
   c->mt = MTables{};
   c->primM = c->mid_primM = c->fluxM = nullptr;
+  for (void* p : c->adj_bufs) (void)hipFree(p);
+  c->adj_bufs.clear();
+  c->am = AdjMesh{};
+  c->adj_steps = c->adj_every = c->adj_nck = c->adj_cap_every = c->adj_cap_nck = 0;
+  c->ckP = c->ckX = c->finP = c->finX = c->tP = c->tX = c->tPm = c->tXm = nullptr;
+  c->tdt = nullptr;
+  c->gP = c->gX = c->ga = c->gb = c->gps = c->gslope = c->lb = c->rb = c->emit = nullptr;
+  c->dtb = c->vmin = nullptr;
+  c->imin = nullptr;
+  c->ext_init = false;
   c->recM = nullptr;
   c->has_mesh = c->has_cells = c->has_ext = c->has_flux = false;
 }
@@ -303,6 +332,46 @@ int get4(rg_ctx* c, double* const dst[4], const D4* src, int n) {
   return RG_OK;
 }
 
+
+// One step of hdsim::TimeAdvance2Heun (hdsim2d.cpp:323-449) on a static
+// mesh: the dt of slot s&1, fluxes, the update to the mid state, fluxes of
+// the mid state, 0.5*(mid + old) and the cell update with the next dt's
+// CFL terms into slot (s+1)&1.  dt_hist[hist] receives the step's dt.
+hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist) {
+  const int cur = s & 1, nxt = (s + 1) & 1;
+  hipError_t e = run_flux(c, c->prim, s, 1);
+  if (e != hipSuccess) return e;
+  UpdArgs a{};
+  a.in = c->ext;
+  a.out = c->mid_ext;
+  a.flux = c->flux;
+  a.cells = c->mid_prim;
+  a.dt_slot = c->words + cur;
+  a.dt_hist = dt_hist;
+  a.step = hist;
+  a.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_CELLS;
+  e = launch(c, KID_UPDATE, [&] {
+    k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c), derr(c, s, 3));
+  });
+  if (e != hipSuccess) return e;
+  e = run_flux(c, c->mid_prim, s, 4);
+  if (e != hipSuccess) return e;
+  UpdArgs b{};
+  b.in = c->mid_ext;
+  b.out = c->ext;
+  b.avg = c->ext;
+  b.flux = c->flux;
+  b.cells = c->prim;
+  b.dt_slot = c->words + cur;
+  b.partials = c->partials;
+  b.step = hist;
+  b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
+  e = launch(c, KID_UPDATE, [&] {
+    k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, b, dparams(c), derr(c, s, 6));
+  });
+  if (e != hipSuccess) return e;
+  return run_reduce_min(c, nxt);
+}
 
 // hdsim::TimeAdvance2Heun with FP32 reconstruction and HLLC and FP64
 // extensives (RG_MIXED).  The FP64 cells are recovered at the end.
@@ -536,6 +605,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     wofs[w + 1] = wofs[w] + 64 * wd;
   }
   const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
+  c->dm_ne = static_cast<int>(ne);
   auto ell_at = [&](int i, int k) {
     return static_cast<size_t>(wofs[i / 64] + i % 64) +
            64 * static_cast<size_t>(k - m->csr_off[i]);
@@ -547,6 +617,56 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       enbr[ell_at(i, k)] = inc[k].nbr;
       einfo[ell_at(i, k)] = inc[k].info;
     }
+  // Adjoint tables.  rev: for each incidence (i, s) with a real neighbour j,
+  // the ELL entry of the same face in j's list.  gf: the faces whose ghost
+  // side copies cell b (b = the ghost's source), with the ghost's side.
+  // ge: the incidences whose neighbour is a ghost copying cell b.
+  std::vector<int> rev(ne, -1);
+  std::vector<int> gf_off(static_cast<size_t>(N) + 1, 0), ge_off(static_cast<size_t>(N) + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    const int a = m->face_first[f], b = m->face_second[f];
+    if (a >= N) ++gf_off[gsrc[a - N] + 1];
+    if (b >= N) ++gf_off[gsrc[b - N] + 1];
+  }
+  for (int i = 0; i < N; ++i)
+    for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
+      const int j = inc[k].nbr;
+      if (j >= N) {
+        ++ge_off[gsrc[j - N] + 1];
+        continue;
+      }
+      const int f = static_cast<int>(inc[k].info & INFO_FACE);
+      int kj = m->csr_off[j];
+      while (kj < m->csr_off[j + 1] && m->csr_face[kj] != f) ++kj;
+      if (kj == m->csr_off[j + 1]) return RG_E_ARG;  // face missing from the neighbour's list
+      rev[ell_at(j, kj)] = static_cast<int>(ell_at(i, k));
+    }
+  for (int i = 0; i < N; ++i) {
+    gf_off[i + 1] += gf_off[i];
+    ge_off[i + 1] += ge_off[i];
+  }
+  std::vector<unsigned> gf(static_cast<size_t>(std::max(gf_off[N], 1)), 0u);
+  std::vector<int> ge(static_cast<size_t>(std::max(ge_off[N], 1)), 0);
+  {
+    std::vector<int> fill(gf_off.begin(), gf_off.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      const int a = m->face_first[f], b = m->face_second[f];
+      if (a >= N) gf[fill[gsrc[a - N]]++] = static_cast<unsigned>(f);
+      if (b >= N) gf[fill[gsrc[b - N]]++] = static_cast<unsigned>(f) | INFO_SECOND;
+    }
+    std::vector<int> gfill(ge_off.begin(), ge_off.end() - 1);
+    for (int i = 0; i < N; ++i)
+      for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k)
+        if (inc[k].nbr >= N) ge[gfill[gsrc[inc[k].nbr - N]]++] = static_cast<int>(ell_at(i, k));
+  }
+  int *rev_d, *gf_off_d, *ge_off_d, *ge_d;
+  unsigned* gf_d;
+  HIPCHK(c, upload(c, &rev_d, rev.data(), ne));
+  HIPCHK(c, upload(c, &gf_off_d, gf_off.data(), gf_off.size()));
+  HIPCHK(c, upload(c, &gf_d, gf.data(), gf.size()));
+  HIPCHK(c, upload(c, &ge_off_d, ge_off.data(), ge_off.size()));
+  HIPCHK(c, upload(c, &ge_d, ge.data(), ge.size()));
+  c->am = AdjMesh{rev_d, gf_off_d, gf_d, ge_off_d, ge_d};
   int *wofs_d, *enbr_d;
   unsigned* einfo_d;
   D2 *eh, *eo;
@@ -715,6 +835,7 @@ int rg_set_cells(rg_ctx* c, const double* d, const double* p, const double* vx, 
   const double* const src[4] = {d, p, vx, vy};
   const int rc = put4(c, c->prim, src, c->N);
   if (rc == RG_OK) c->has_cells = true;
+  c->ext_init = false;
   return rc;
 }
 
@@ -730,6 +851,7 @@ int rg_set_extensives(rg_ctx* c, const double* m, const double* E, const double*
   const double* const src[4] = {m, E, Mx, My};
   const int rc = put4(c, c->ext, src, c->N);
   if (rc == RG_OK) c->has_ext = true;
+  c->ext_init = false;
   return rc;
 }
 
@@ -755,6 +877,7 @@ int rg_init_extensives(rg_ctx* c) {
   }));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->has_ext = true;
+  c->ext_init = true;
   return RG_OK;
 }
 
@@ -828,6 +951,7 @@ int rg_update_extensives(rg_ctx* c, double dt) {
   a.flux = c->flux;
   a.dt = dt;
   a.flags = UPD_FACES;
+  c->ext_init = false;
   HIPCHK(c, run_update(c, a, 0, 3));
   return collect_error(c);
 }
@@ -839,6 +963,7 @@ int rg_update_cells(rg_ctx* c) {
   a.out = c->ext;
   a.cells = c->prim;
   a.flags = UPD_CELLS;
+  c->ext_init = false;
   HIPCHK(c, run_update(c, a, 0, 3));
   const int rc = collect_error(c);
   if (rc == RG_OK) c->has_cells = true;
@@ -857,6 +982,7 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
     HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->dt_hist), sizeof(double) * nsteps));
     c->dt_hist_cap = nsteps;
   }
+  c->ext_init = false;
   if (c->prm.precision == RG_MIXED && integrator == 2 && c->prm.order == 2)
     return advance_mixed(c, nsteps, dt_out);
   // first dt: SimpleCFL on the current cells into slot 0
@@ -879,37 +1005,192 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
       HIPCHK(c, run_update(c, a, s, 3));
       HIPCHK(c, run_reduce_min(c, nxt));
     } else {
-      // hdsim::TimeAdvance2Heun, stage 1
-      HIPCHK(c, run_flux(c, c->prim, s, 1));
-      UpdArgs a{};
-      a.in = c->ext;
-      a.out = c->mid_ext;
-      a.flux = c->flux;
-      a.cells = c->mid_prim;
-      a.dt_slot = c->words + cur;
-      a.dt_hist = dt_out ? c->dt_hist : nullptr;
-      a.step = s;
-      a.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_CELLS;
-      HIPCHK(c, run_update(c, a, s, 3));
-      // stage 2 on the mid cells, then 0.5*(mid + old) and the cell update
-      HIPCHK(c, run_flux(c, c->mid_prim, s, 4));
-      UpdArgs b{};
-      b.in = c->mid_ext;
-      b.out = c->ext;
-      b.avg = c->ext;
-      b.flux = c->flux;
-      b.cells = c->prim;
-      b.dt_slot = c->words + cur;
-      b.partials = c->partials;
-      b.step = s;
-      b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
-      HIPCHK(c, run_update(c, b, s, 6));
-      HIPCHK(c, run_reduce_min(c, nxt));
+      HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s));
     }
   }
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
+  return collect_error(c);
+}
+
+int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out) {
+  if (!ready(c, true, true) || c->prm.order != 2 || c->prm.precision != RG_FP64 || c->dm.wx ||
+      nsteps <= 0 || nsteps >= (1 << 24) || every <= 0 || !c->ext_init)
+    return RG_E_ARG;
+  const int N = c->N, F = c->F;
+  const int nck = (nsteps + every - 1) / every, ev = std::min(every, nsteps);
+  // buffers sized per run (reallocated when a run needs more)
+  if (nck > c->adj_cap_nck || ev > c->adj_cap_every || !c->gP) {
+    for (void* q : c->adj_bufs) (void)hipFree(q);
+    c->adj_bufs.clear();
+    auto get = [&](auto** ptr, size_t n) -> hipError_t {
+      void* q = nullptr;
+      const hipError_t e = hipMalloc(&q, sizeof(**ptr) * std::max<size_t>(n, 1));
+      if (e == hipSuccess) {
+        c->adj_bufs.push_back(q);
+        *ptr = static_cast<std::remove_reference_t<decltype(*ptr)>>(q);
+      }
+      return e;
+    };
+    const size_t n = static_cast<size_t>(N), ne = static_cast<size_t>(c->dm_ne);
+    HIPCHK(c, get(&c->ckP, n * nck));
+    HIPCHK(c, get(&c->ckX, n * nck));
+    HIPCHK(c, get(&c->finP, n));
+    HIPCHK(c, get(&c->finX, n));
+    HIPCHK(c, get(&c->tP, n * ev));
+    HIPCHK(c, get(&c->tX, n * ev));
+    HIPCHK(c, get(&c->tPm, n * ev));
+    HIPCHK(c, get(&c->tXm, n * ev));
+    HIPCHK(c, get(&c->tdt, static_cast<size_t>(ev)));
+    HIPCHK(c, get(&c->gP, n));
+    HIPCHK(c, get(&c->gX, n));
+    HIPCHK(c, get(&c->ga, n));
+    HIPCHK(c, get(&c->gb, n));
+    HIPCHK(c, get(&c->gps, n));
+    HIPCHK(c, get(&c->gslope, 2 * n));
+    HIPCHK(c, get(&c->lb, static_cast<size_t>(F)));
+    HIPCHK(c, get(&c->rb, static_cast<size_t>(F)));
+    HIPCHK(c, get(&c->emit, ne));
+    HIPCHK(c, get(&c->dtb, 2 * static_cast<size_t>(blocks_for(F))));
+    HIPCHK(c, get(&c->vmin, static_cast<size_t>(blocks_for(N))));
+    HIPCHK(c, get(&c->imin, static_cast<size_t>(blocks_for(N))));
+    c->adj_cap_nck = nck;
+    c->adj_cap_every = ev;
+  }
+  if (dt_out && c->dt_hist_cap < nsteps) {
+    if (c->dt_hist) (void)hipFree(c->dt_hist);
+    c->dt_hist = nullptr;
+    c->dt_hist_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->dt_hist), sizeof(double) * nsteps));
+    c->dt_hist_cap = nsteps;
+  }
+  const size_t bytes = sizeof(D4) * static_cast<size_t>(N);
+  c->ext_init = false;
+  HIPCHK(c, run_cfl(c, c->prim, 0));
+  for (int s = 0; s < nsteps; ++s) {
+    if (s % every == 0) {
+      const size_t k = static_cast<size_t>(s / every) * N;
+      HIPCHK(c, hipMemcpyAsync(c->ckP + k, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->ckX + k, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->finP, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->finX, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (dt_out)
+    HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
+                             c->stream));
+  const int rc = collect_error(c);
+  if (rc != RG_OK) return rc;
+  c->adj_steps = nsteps;
+  c->adj_every = every;
+  c->adj_nck = nck;
+  return RG_OK;
+}
+
+namespace {
+
+// Reverse of one Heun stage on the cells q (P_n or P_mid of step t of the
+// tape): seed c->ga = adjoint of the stage's output extensives.  mode 2
+// (second stage) leaves the adjoint of the mid extensives in c->ga; mode 1
+// (first stage) writes the step-start adjoints to c->gP and c->gX.
+hipError_t adj_stage(rg_ctx* c, const D4* q, const D4* X, const double* dtp, double* dtb,
+                     int mode, int step) {
+  const int N = c->N, F = c->F;
+  hipError_t e = run_slopes(c, q, step, mode == 2 ? 4 : 1);
+  if (e != hipSuccess) return e;
+  e = launch(c, KID_ADJ_FLUX, [&] {
+    k_adj_flux<<<blocks_for(F), kBlock, 0, c->stream>>>(c->dm, q, c->rec, c->ga, dtp, c->prm.cfl,
+                                                         c->lb, c->rb, dtb, c->prm.gamma);
+  });
+  if (e != hipSuccess) return e;
+  e = launch(c, KID_ADJ_CELL, [&] {
+    k_adj_cellstate<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->am, c->lb, c->rb, c->gps,
+                                                              c->gslope);
+  });
+  if (e != hipSuccess) return e;
+  e = launch(c, KID_ADJ_SLOPES, [&] {
+    k_adj_slopes<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, q, c->gslope, c->gps, c->emit,
+                                                           dparams(c));
+  });
+  if (e != hipSuccess) return e;
+  return launch(c, KID_ADJ_CELL, [&] {
+    k_adj_gather<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->am, c->gps, c->emit, mode, X,
+                                                           c->ga, c->gb, c->gP, c->gX,
+                                                           c->prm.gamma);
+  });
+}
+
+}  // namespace
+
+int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, double* gcells0) {
+  if (!c || !c->has_mesh || c->adj_steps <= 0 || !gcells || !gcells0) return RG_E_ARG;
+  const int N = c->N, F = c->F, nsteps = c->adj_steps, every = c->adj_every;
+  const size_t n = static_cast<size_t>(N), bytes = sizeof(D4) * n;
+  const unsigned nbF = blocks_for(F), nbN = blocks_for(N);
+  {
+    const double* const src[4] = {gcells, gcells + n, gcells + 2 * n, gcells + 3 * n};
+    const int rc = put4(c, c->gP, src, N);
+    if (rc != RG_OK) return rc;
+  }
+  if (gext) {
+    const double* const src[4] = {gext, gext + n, gext + 2 * n, gext + 3 * n};
+    const int rc = put4(c, c->gX, src, N);
+    if (rc != RG_OK) return rc;
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->gX, 0, bytes, c->stream));
+  }
+  for (int seg = c->adj_nck - 1; seg >= 0; --seg) {
+    const int k0 = seg * every, k1 = std::min(k0 + every, nsteps), L = k1 - k0;
+    // recompute the segment from its checkpoint, taping each step's states
+    HIPCHK(c, hipMemcpyAsync(c->prim, c->ckP + static_cast<size_t>(seg) * n, bytes,
+                             hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->ext, c->ckX + static_cast<size_t>(seg) * n, bytes,
+                             hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, run_cfl(c, c->prim, k0 & 1));
+    for (int t = 0; t < L; ++t) {
+      const size_t o = static_cast<size_t>(t) * n;
+      HIPCHK(c, hipMemcpyAsync(c->tP + o, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->tX + o, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, heun_step(c, k0 + t, c->tdt, t));
+      HIPCHK(c, hipMemcpyAsync(c->tPm + o, c->mid_prim, bytes, hipMemcpyDeviceToDevice,
+                               c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->tXm + o, c->mid_ext, bytes, hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    // reverse through the segment; c->ext holds X at the segment's end
+    for (int t = L - 1; t >= 0; --t) {
+      const size_t o = static_cast<size_t>(t) * n;
+      const D4* Xn1 = (t + 1 < L) ? c->tX + o + n : c->ext;
+      HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
+        k_adj_recover<<<nbN, kBlock, 0, c->stream>>>(c->dm, Xn1, c->gP, c->gX, c->ga, c->gb,
+                                                     c->prm.gamma);
+      }));
+      HIPCHK(c, adj_stage(c, c->tPm + o, c->tXm + o, c->tdt + t, c->dtb, 2, k0 + t));
+      HIPCHK(c, adj_stage(c, c->tP + o, nullptr, c->tdt + t, c->dtb + nbF, 1, k0 + t));
+      HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
+        k_adj_argmin<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->tP + o, c->prm.gamma, c->vmin,
+                                                    c->imin);
+      }));
+      HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
+        k_adj_dt<<<1, kBlock, 0, c->stream>>>(c->dm, c->tP + o, c->dtb, static_cast<int>(2 * nbF),
+                                              c->vmin, c->imin, static_cast<int>(nbN),
+                                              c->prm.cfl, c->prm.gamma, c->gP);
+      }));
+    }
+  }
+  HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
+    k_adj_init<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->ckP, c->gX, c->gP, c->prm.gamma);
+  }));
+  {
+    double* const dst[4] = {gcells0, gcells0 + n, gcells0 + 2 * n, gcells0 + 3 * n};
+    const int rc = get4(c, dst, c->gP, N);
+    if (rc != RG_OK) return rc;
+  }
+  // leave the forward run's final state in place
+  HIPCHK(c, hipMemcpyAsync(c->prim, c->finP, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ext, c->finX, bytes, hipMemcpyDeviceToDevice, c->stream));
   return collect_error(c);
 }
 

@@ -70,6 +70,8 @@ def lib():
     L.rg_update_extensives.argtypes = [vp, ctypes.c_double]
     L.rg_update_cells.argtypes = [vp]
     L.rg_advance.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.rg_adjoint_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.rg_adjoint_backward.argtypes = [vp, vp, vp, vp]
     L.rg_last_error.argtypes = [vp, P(Error)]
     L.rg_profile_enable.argtypes = [vp, ctypes.c_int32]
     L.rg_kernel_count.restype = ctypes.c_int32
@@ -202,6 +204,22 @@ class RichGPU:
         dts = np.empty(max(nsteps, 1)) if record_dt else None
         self._check(self._lib.rg_advance(self._h, integrator, nsteps, _p(dts)))
         return dts[:nsteps] if record_dt else None
+
+    def adjoint_forward(self, nsteps, every=20, record_dt=True):
+        """rg_advance(nsteps, Heun) keeping device checkpoints every `every`
+        steps for adjoint_backward; needs init_extensives() of the cells."""
+        dts = np.empty(max(nsteps, 1)) if record_dt else None
+        self._check(self._lib.rg_adjoint_forward(self._h, nsteps, every, _p(dts)))
+        return dts[:nsteps] if record_dt else None
+
+    def adjoint_backward(self, gcells, gext=None):
+        """d(loss)/d(initial cells) (4, N) from d(loss)/d(final cells) (4, N)
+        (and optionally d(loss)/d(final extensives))."""
+        g = self._rows(gcells, self.N)
+        x = None if gext is None else self._rows(gext, self.N)
+        out = np.empty((4, self.N))
+        self._check(self._lib.rg_adjoint_backward(self._h, _p(g), _p(x), _p(out)))
+        return out
 
     def set_stream(self, stream_ptr):
         self._check(self._lib.rg_set_stream(self._h, ctypes.c_void_p(stream_ptr)))

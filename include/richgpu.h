@@ -149,6 +149,23 @@ int rg_update_cells(rg_ctx* ctx);
  * [nsteps] receives the dts (may be NULL).  Synchronises once at the end. */
 int rg_advance(rg_ctx* ctx, int32_t integrator, int32_t nsteps, double* dt_out);
 
+/* adjoint of the fixed-topology step ----------------------------------- */
+/* No counterpart in the reference (RICH has no derivatives): the reverse
+ * mode of rg_advance(ctx, 2, nsteps) for order 2, RG_FP64, zero face
+ * velocities, for gradient-based callers (inverse problems, sensitivity
+ * studies).  rg_adjoint_forward runs the same steps as rg_advance and keeps
+ * a device checkpoint of (cells, extensives) every `every` steps; the
+ * extensives must be rg_init_extensives of the cells (else RG_E_ARG). */
+int rg_adjoint_forward(rg_ctx* ctx, int32_t nsteps, int32_t every, double* dt_out);
+/* Given d(loss)/d(final cells) gcells [4][N] (density, pressure, vx, vy)
+ * and optionally d(loss)/d(final extensives) gext [4][N] (NULL = 0),
+ * writes d(loss)/d(initial cells) gcells0 [4][N], through init_extensives.
+ * Each segment between checkpoints is recomputed once; the derivative is
+ * that of the branches the forward values select (HLLC wave pattern,
+ * limiter choices, shock switch, CFL argmin).  The final state of the
+ * forward run is left in the context. */
+int rg_adjoint_backward(rg_ctx* ctx, const double* gcells, const double* gext, double* gcells0);
+
 /* diagnostics -------------------------------------------------------- */
 int rg_last_error(rg_ctx* ctx, rg_error* err);
 /* Per-kernel HIP-event timing of every launch on the context's stream. */

@@ -1,0 +1,101 @@
+"""GPU adjoint (rg_adjoint_forward / rg_adjoint_backward) against the CPU
+forward-mode reference (oracle/tangent.cpp, dual numbers over the same
+step) through the identity <J^T w, v> = <w, J v>: the adjoint of a random
+final-state weight w dotted with a random initial perturbation v equals w
+dotted with the tangent of v.  Both follow the branches the forward values
+select, so they agree to round-off (bar: 1e-9 relative).  The forward run
+under the adjoint is itself bit-identical to the oracle."""
+import numpy as np
+import pytest
+
+import cases
+import pyoracle as po
+from test_oracle_tangent import smooth_cells
+
+pytestmark = pytest.mark.gpu
+
+MESHES = dict(cases.small_meshes())
+
+
+def _adjoint(m, cells, nsteps, every, w, wx=None, slf=True):
+    from rich_amd.gpu import RichGPU
+    g = RichGPU(m, order=2, slf=slf)
+    g.set_cells(cells)
+    g.init_extensives()
+    g.adjoint_forward(nsteps, every)
+    final = g.get_cells()
+    gc0 = g.adjoint_backward(w, wx)
+    again = g.get_cells()
+    g.close()
+    return final, gc0, again
+
+
+def _check(m, cells, nsteps, every, slf=True, K=3, seed=0, with_ext=False):
+    rng = np.random.default_rng(seed)
+    N = m.n_cells
+    w = rng.standard_normal((4, N))
+    wx = rng.standard_normal((4, N)) if with_ext else None
+    v = rng.standard_normal((K, 4, N)) * np.abs(cells)[None]
+    final, gc0, again = _adjoint(m, cells, nsteps, every, w, wx, slf)
+    o = po.Oracle(m, order=2, slf=slf)
+    cT, dcT, xT, dxT, _ = o.run_tangent(cells, v, nsteps)
+    np.testing.assert_array_equal(final, cT)
+    np.testing.assert_array_equal(again, final)
+    errs = []
+    for k in range(K):
+        lhs = np.sum(gc0 * v[k])
+        rhs = np.sum(w * dcT[k]) + (np.sum(wx * dxT[k]) if with_ext else 0.0)
+        errs.append(abs(lhs - rhs) / max(abs(rhs), 1e-300))
+    print(f"N={N} steps={nsteps} every={every} slf={slf} dot-product rel err max {max(errs):.2e}")
+    assert max(errs) < 1e-9
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_adjoint_dot_product_small_meshes(name):
+    m = MESHES[name]
+    _check(m, cases.rough_cells(m, seed=31), nsteps=5, every=2)
+
+
+@pytest.mark.parametrize("name", ["rigid", "periodic"])
+def test_adjoint_without_limiter(name):
+    m = MESHES[name]
+    _check(m, smooth_cells(m), nsteps=4, every=3, slf=False)
+
+
+def test_adjoint_with_extensive_seed():
+    m = MESHES["periodic_flip"]
+    _check(m, cases.rough_cells(m, seed=5), nsteps=3, every=1, with_ext=True)
+
+
+def test_adjoint_sedov_shocked():
+    """Strong blast: the shocked limiter and every HLLC wave pattern occur."""
+    m = cases.sedov_mesh(32)
+    _check(m, cases.sedov_cells(m), nsteps=30, every=7)
+
+
+def test_adjoint_matches_gpu_finite_differences_smooth():
+    """Where the step is smooth (no limiter), central differences of GPU
+    forward runs agree with the adjoint."""
+    from rich_amd.gpu import RichGPU
+    m = MESHES["periodic"]
+    cells = smooth_cells(m)
+    rng = np.random.default_rng(4)
+    w = rng.standard_normal((4, m.n_cells))
+    _, gc0, _ = _adjoint(m, cells, 6, 4, w, slf=False)
+
+    def loss(c):
+        g = RichGPU(m, order=2, slf=False)
+        g.set_cells(c)
+        g.init_extensives()
+        g.advance(6, integrator=2, record_dt=False)
+        out = float(np.sum(w * g.get_cells()))
+        g.close()
+        return out
+
+    for k in range(3):
+        v = rng.standard_normal((4, m.n_cells))
+        h = 1e-6
+        fd = (loss(cells + h * v) - loss(cells - h * v)) / (2 * h)
+        an = float(np.sum(gc0 * v))
+        print("FD", fd, "adjoint", an)
+        assert abs(fd - an) <= 1e-6 * abs(an)
