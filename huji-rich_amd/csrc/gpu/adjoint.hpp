@@ -54,18 +54,77 @@ __device__ __forceinline__ void block_sum(double v, double* out) {
   if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
+// The dt of a step, dt = cfl * min_i inv_i(P), has its adjoint applied one
+// kernel late, by the next kernel that reads the step's cell adjoint: the
+// face shares of d(loss)/d(dt) (dtb, both stages) and the per-block argmin
+// partials are reduced by every block in the same fixed order.
+struct PendingDt {
+  const D4* q;  // cells the dt was computed from (null: none pending)
+  const double* dtb;
+  int ndtb;
+  const double* vmin;
+  const int* imin;
+  int nmin;
+  double cfl;
+};
+
+// Block-cooperative (every thread of the block calls it): the argmin cell
+// and the adjoint it receives.
+__device__ __forceinline__ void pending_dt(const DMesh& m, const PendingDt& pd, double gamma,
+                                           int& cell, Cell& cb) {
+  __shared__ double ssum[256];
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  double acc_v = 0;
+  for (int k = threadIdx.x; k < pd.ndtb; k += 256) acc_v += pd.dtb[k];
+  double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  int ix = 0x7fffffff;
+  for (int k = threadIdx.x; k < pd.nmin; k += 256) {
+    if (pd.vmin[k] < v || (pd.vmin[k] == v && pd.imin[k] < ix)) {
+      v = pd.vmin[k];
+      ix = pd.imin[k];
+    }
+  }
+  ssum[threadIdx.x] = acc_v;
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = ix;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) {
+      ssum[threadIdx.x] += ssum[threadIdx.x + s];
+      const double o = sv[threadIdx.x + s];
+      const int oi = si[threadIdx.x + s];
+      if (o < sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = o;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  cell = si[0];
+  cb = Cell{0, 0, 0, 0};
+  if (cell >= 0 && cell < m.N)
+    cfl_inv_vjp(gamma, m.width[cell], ld_cell(pd.q, cell), pd.cfl * ssum[0], cb);
+}
+
 // Reverse of P_{n+1} = recover(X_{n+1}) and of the Heun average
-// X_{n+1} = 0.5 (X_tmp + X_n): ga = gb = 0.5 (gX + recover^T gP).
+// X_{n+1} = 0.5 (X_tmp + X_n): ga = gb = 0.5 (gX + recover^T gP), after the
+// pending dt adjoint of step n+1 is added to gP.
 __global__ void __launch_bounds__(256) k_adj_recover(DMesh m, const D4* __restrict__ X,
                                                      const D4* __restrict__ gP,
                                                      const D4* __restrict__ gX,
                                                      D4* __restrict__ ga, D4* __restrict__ gb,
-                                                     double gamma) {
+                                                     double gamma, PendingDt pd) {
+  int dcell = -1;
+  Cell dcb{0, 0, 0, 0};
+  if (pd.q) pending_dt(m, pd, gamma, dcell, dcb);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const D4 x = X[i], g = gX[i];
+  Cell cb = d4_cell(gP[i]);
+  if (i == dcell) acc(cb, dcb);
   double mb, Eb, Mxb, Myb;
-  recover_vjp(gamma, m.vol[i], x.a, x.b, x.c, x.d, d4_cell(gP[i]), mb, Eb, Mxb, Myb);
+  recover_vjp(gamma, m.vol[i], x.a, x.b, x.c, x.d, cb, mb, Eb, Mxb, Myb);
   const D4 h{0.5 * (g.a + mb), 0.5 * (g.b + Eb), 0.5 * (g.c + Mxb), 0.5 * (g.d + Myb)};
   ga[i] = h;
   gb[i] = h;
@@ -107,17 +166,12 @@ __global__ void __launch_bounds__(256) k_adj_flux(DMesh m, const D4* __restrict_
   block_sum(dtb, dtb_partials);
 }
 
-// Per cell: adjoints of the cell value and of its slopes from the face
-// states it produced (its own faces, and the ghost sides that copy it).
-__global__ void __launch_bounds__(256) k_adj_cellstate(DMesh m, AdjMesh am,
-                                                       const D4* __restrict__ lb,
-                                                       const D4* __restrict__ rb,
-                                                       D4* __restrict__ gps,
-                                                       D4* __restrict__ gslope) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.N) return;
-  const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
-  Cell v{0, 0, 0, 0}, sx{0, 0, 0, 0}, sy{0, 0, 0, 0};
+// Adjoints of cell i's value (v) and slopes (sx, sy) from the face states
+// it produced (its own faces, and the ghost sides that copy it).
+__device__ __forceinline__ void cell_state_adj(const DMesh& m, const AdjMesh& am, const D4* lb,
+                                               const D4* rb, int i, int deg, int eb, Cell& v,
+                                               Cell& sx, Cell& sy) {
+  v = sx = sy = Cell{0, 0, 0, 0};
   for (int s = 0; s < deg; ++s) {
     const unsigned info = m.einfo[eb + 64 * s];
     const int f = static_cast<int>(info & INFO_FACE);
@@ -147,9 +201,6 @@ __global__ void __launch_bounds__(256) k_adj_cellstate(DMesh m, AdjMesh am,
       acc(v, g);
     }
   }
-  gps[i] = cell_d4(v);
-  gslope[2 * i] = cell_d4(sx);
-  gslope[2 * i + 1] = cell_d4(sy);
 }
 
 // one field of the limiter with the choice recorded: psi = min(psi, ratio)
@@ -196,21 +247,26 @@ __device__ __forceinline__ void fld_add(Cell& c, int k, double v) {
   else c.vy += v;
 }
 
-// Reverse of calc_slope (LinearGaussImproved.cpp:425-494): given the
-// adjoint of the cell's limited slopes, adds the cell's own share to gps and
-// writes each incidence's share for the neighbour to emit[ELL entry].
-__global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, const D4* __restrict__ q,
-                                                    const D4* __restrict__ gslope,
+// Per cell: the adjoint of the face states it produced (cell_state_adj),
+// then the reverse of calc_slope (LinearGaussImproved.cpp:425-494) for the
+// slope part: writes the cell's own share to gps and each incidence's share
+// for the neighbour to emit[ELL entry].
+__global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D4* __restrict__ q,
+                                                    const D4* __restrict__ lbf,
+                                                    const D4* __restrict__ rbf,
                                                     D4* __restrict__ gps, D4* __restrict__ emit,
                                                     DParams prm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
+  Cell vstate, gx, gy;
+  cell_state_adj(m, am, lbf, rbf, i, deg, eb, vstate, gx, gy);
   const Cell cell = ld_cell(q, i);
   const CellGeo g = m.cgeo[i];
   const int kind = static_cast<int>(g.kind);
   if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
     for (int s = 0; s < deg; ++s) emit[eb + 64 * s] = D4{0, 0, 0, 0};
+    gps[i] = cell_d4(vstate);
     return;
   }
   const bool boundary = kind == KIND_BOUNDARY;
@@ -302,8 +358,7 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, const D4* __restric
     }
   }
   // ---- reverse: limited slope = psi * unlimited slope (per field)
-  const Cell gx = d4_cell(gslope[2 * i]), gy = d4_cell(gslope[2 * i + 1]);
-  Cell gsx, gsy, self{0, 0, 0, 0};
+  Cell gsx, gsy, self = vstate;
   int tgt[4] = {-1, -1, -1, -1};
   double tval[4] = {0, 0, 0, 0};
   for (int k = 0; k < 4; ++k) {
@@ -376,67 +431,58 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, const D4* __restric
       if (tgt[k] == s) fld_add(e, k, tval[k]);
     emit[eb + 64 * s] = cell_d4(e);
   }
-  Cell v = d4_cell(gps[i]);
-  acc(v, self);
-  gps[i] = cell_d4(v);
+  gps[i] = cell_d4(self);
 }
 
 // Per cell: total adjoint of the stage's cell value (own share, the shares
 // its neighbours emitted for it, and those emitted for ghosts that copy it),
 // then either the recovery reverse (mode 2: ga += recover^T gp, X = the
 // stage's extensives) or the step-start outputs (mode 1: gP = gp,
-// gX = gb + ga).
+// gX = gb + ga) with the per-block CFL argmin of the step-start cells q.
 __global__ void __launch_bounds__(256) k_adj_gather(DMesh m, AdjMesh am,
                                                     const D4* __restrict__ gps,
                                                     const D4* __restrict__ emit, int mode,
                                                     const D4* __restrict__ X, D4* ga,
                                                     const D4* __restrict__ gb, D4* gP, D4* gX,
-                                                    double gamma) {
+                                                    const D4* __restrict__ q, double* vmin,
+                                                    int* imin, double gamma) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.N) return;
-  Cell v = d4_cell(gps[i]);
-  const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
-  for (int s = 0; s < deg; ++s) {
-    const int r = am.rev[eb + 64 * s];
-    if (r >= 0) acc(v, d4_cell(emit[r]));
-  }
-  for (int k = am.ge_off[i]; k < am.ge_off[i + 1]; ++k) {
-    const int idx = am.ge[k];
-    Cell e = d4_cell(emit[idx]);
-    if (m.bc == BC_RIGID) {
-      const D2 n = m.gn[m.enbr[idx] - m.N];
-      reflect_vec(n.x, n.y, e.vx, e.vy);
-    }
-    acc(v, e);
-  }
-  if (mode == 2) {
-    const D4 x = X[i];
-    double mb, Eb, Mxb, Myb;
-    recover_vjp(gamma, m.vol[i], x.a, x.b, x.c, x.d, v, mb, Eb, Mxb, Myb);
-    const D4 s = ga[i];
-    ga[i] = D4{s.a + mb, s.b + Eb, s.c + Mxb, s.d + Myb};
-  } else {
-    gP[i] = cell_d4(v);
-    const D4 a = ga[i], b = gb[i];
-    gX[i] = D4{a.a + b.a, a.b + b.b, a.c + b.c, a.d + b.d};
-  }
-}
-
-// Per block: the smallest CFL inverse time step and its cell (lowest index
-// on ties).
-__global__ void __launch_bounds__(256) k_adj_argmin(DMesh m, const D4* __restrict__ q,
-                                                    double gamma, double* __restrict__ vmin,
-                                                    int* __restrict__ imin) {
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  double inv = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   int ix = 0x7fffffff;
   if (i < m.N) {
-    v = cfl_inv(m, i, gamma, m.width[i], ld_cell(q, i));
-    ix = i;
+    Cell v = d4_cell(gps[i]);
+    const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
+    for (int s = 0; s < deg; ++s) {
+      const int r = am.rev[eb + 64 * s];
+      if (r >= 0) acc(v, d4_cell(emit[r]));
+    }
+    for (int k = am.ge_off[i]; k < am.ge_off[i + 1]; ++k) {
+      const int idx = am.ge[k];
+      Cell e = d4_cell(emit[idx]);
+      if (m.bc == BC_RIGID) {
+        const D2 n = m.gn[m.enbr[idx] - m.N];
+        reflect_vec(n.x, n.y, e.vx, e.vy);
+      }
+      acc(v, e);
+    }
+    if (mode == 2) {
+      const D4 x = X[i];
+      double mb, Eb, Mxb, Myb;
+      recover_vjp(gamma, m.vol[i], x.a, x.b, x.c, x.d, v, mb, Eb, Mxb, Myb);
+      const D4 s = ga[i];
+      ga[i] = D4{s.a + mb, s.b + Eb, s.c + Mxb, s.d + Myb};
+    } else {
+      gP[i] = cell_d4(v);
+      const D4 a = ga[i], b = gb[i];
+      gX[i] = D4{a.a + b.a, a.b + b.b, a.c + b.c, a.d + b.d};
+      inv = cfl_inv(m, i, gamma, m.width[i], ld_cell(q, i));
+      ix = i;
+    }
   }
-  sv[threadIdx.x] = v;
+  if (mode != 1) return;  // uniform across the grid
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  sv[threadIdx.x] = inv;
   si[threadIdx.x] = ix;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -456,59 +502,19 @@ __global__ void __launch_bounds__(256) k_adj_argmin(DMesh m, const D4* __restric
   }
 }
 
-// One block of 256: d(loss)/d(dt) = sum of the face shares of both stages
-// (fixed order), dt = cfl * min_i inv_i, and the reverse of inv at the
-// argmin cell into gP.
-__global__ void __launch_bounds__(256) k_adj_dt(DMesh m, const D4* __restrict__ q,
-                                                const double* __restrict__ dtb, int ndtb,
-                                                const double* __restrict__ vmin,
-                                                const int* __restrict__ imin, int nmin,
-                                                double cfl, double gamma, D4* gP) {
-  __shared__ double ssum[256];
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  double acc_v = 0;
-  for (int k = threadIdx.x; k < ndtb; k += 256) acc_v += dtb[k];
-  double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
-  int ix = 0x7fffffff;
-  for (int k = threadIdx.x; k < nmin; k += 256) {
-    if (vmin[k] < v || (vmin[k] == v && imin[k] < ix)) {
-      v = vmin[k];
-      ix = imin[k];
-    }
-  }
-  ssum[threadIdx.x] = acc_v;
-  sv[threadIdx.x] = v;
-  si[threadIdx.x] = ix;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (static_cast<int>(threadIdx.x) < s) {
-      ssum[threadIdx.x] += ssum[threadIdx.x + s];
-      const double o = sv[threadIdx.x + s];
-      const int oi = si[threadIdx.x + s];
-      if (o < sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
-        sv[threadIdx.x] = o;
-        si[threadIdx.x] = oi;
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && si[0] >= 0 && si[0] < m.N) {
-    const int c = si[0];
-    Cell cb = d4_cell(gP[c]);
-    cfl_inv_vjp(gamma, m.width[c], ld_cell(q, c), cfl * ssum[0], cb);
-    gP[c] = cell_d4(cb);
-  }
-}
-
-// Reverse of init_extensives: gP += init^T gX.
+// Reverse of init_extensives, gP += init^T gX, after the pending dt adjoint
+// of step 0 is added.
 __global__ void __launch_bounds__(256) k_adj_init(DMesh m, const D4* __restrict__ q,
                                                   const D4* __restrict__ gX, D4* gP,
-                                                  double gamma) {
+                                                  double gamma, PendingDt pd) {
+  int dcell = -1;
+  Cell dcb{0, 0, 0, 0};
+  if (pd.q) pending_dt(m, pd, gamma, dcell, dcb);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const D4 x = gX[i];
   Cell cb = d4_cell(gP[i]);
+  if (i == dcell) acc(cb, dcb);
   init_vjp(gamma, m.vol[i], ld_cell(q, i), x.a, x.b, x.c, x.d, cb);
   gP[i] = cell_d4(cb);
 }

@@ -105,9 +105,10 @@ struct rg_ctx {
   std::vector<void*> adj_bufs;  // checkpoints and tape (sized per run)
   D4 *ckP = nullptr, *ckX = nullptr, *finP = nullptr, *finX = nullptr;
   D4 *tP = nullptr, *tX = nullptr, *tPm = nullptr, *tXm = nullptr;
+  Rec *tR1 = nullptr, *tR2 = nullptr;
   double* tdt = nullptr;
   D4 *gP = nullptr, *gX = nullptr, *ga = nullptr, *gb = nullptr, *gps = nullptr;
-  D4 *gslope = nullptr, *lb = nullptr, *rb = nullptr, *emit = nullptr;
+  D4 *lb = nullptr, *rb = nullptr, *emit = nullptr;
   double *dtb = nullptr, *vmin = nullptr;
   int* imin = nullptr;
 
@@ -171,8 +172,9 @@ void free_mesh(rg_ctx* c) {
   c->am = AdjMesh{};
   c->adj_steps = c->adj_every = c->adj_nck = c->adj_cap_every = c->adj_cap_nck = 0;
   c->ckP = c->ckX = c->finP = c->finX = c->tP = c->tX = c->tPm = c->tXm = nullptr;
+  c->tR1 = c->tR2 = nullptr;
   c->tdt = nullptr;
-  c->gP = c->gX = c->ga = c->gb = c->gps = c->gslope = c->lb = c->rb = c->emit = nullptr;
+  c->gP = c->gX = c->ga = c->gb = c->gps = c->lb = c->rb = c->emit = nullptr;
   c->dtb = c->vmin = nullptr;
   c->imin = nullptr;
   c->ext_init = false;
@@ -268,21 +270,23 @@ bool ready(const rg_ctx* c, bool need_cells, bool need_ext) {
   return c && c->has_mesh && (!need_cells || c->has_cells) && (!need_ext || c->has_ext);
 }
 
-hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase) {
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out = nullptr) {
+  Rec* r = out ? out : c->rec;
   return launch(c, KID_SLOPES, [&] {
-    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, c->rec, dparams(c),
+    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, r, dparams(c),
                                                          derr(c, step, phase));
   });
 }
 
-// FluxCalculator on `cells` into c->flux.
-hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base) {
+// FluxCalculator on `cells` into c->flux (slopes into `rec`, default c->rec).
+hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* rec = nullptr) {
   if (c->prm.order == 2) {
-    const hipError_t e = run_slopes(c, cells, step, phase_base);
+    Rec* r = rec ? rec : c->rec;
+    const hipError_t e = run_slopes(c, cells, step, phase_base, r);
     if (e != hipSuccess) return e;
     return launch(c, KID_FLUX2, [&] {
-      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, c->rec, c->flux,
-                                                           dparams(c), derr(c, step, phase_base + 1));
+      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, r, c->flux, dparams(c),
+                                                           derr(c, step, phase_base + 1));
     });
   }
   return launch(c, KID_FLUX1, [&] {
@@ -333,19 +337,36 @@ int get4(rg_ctx* c, double* const dst[4], const D4* src, int n) {
 }
 
 
+// State buffers of one Heun step: start (P, X), mid (Pm, Xm), end (Po, Xo),
+// and where each stage's slopes go.  The default is the context's state
+// with the end overwriting the start.
+struct HeunBufs {
+  const D4* P;
+  const D4* X;
+  D4* Pm;
+  D4* Xm;
+  D4* Po;
+  D4* Xo;
+  Rec* r1;
+  Rec* r2;
+};
+HeunBufs ctx_bufs(rg_ctx* c) {
+  return HeunBufs{c->prim, c->ext, c->mid_prim, c->mid_ext, c->prim, c->ext, c->rec, c->rec};
+}
+
 // One step of hdsim::TimeAdvance2Heun (hdsim2d.cpp:323-449) on a static
 // mesh: the dt of slot s&1, fluxes, the update to the mid state, fluxes of
 // the mid state, 0.5*(mid + old) and the cell update with the next dt's
 // CFL terms into slot (s+1)&1.  dt_hist[hist] receives the step's dt.
-hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist) {
+hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs& h) {
   const int cur = s & 1, nxt = (s + 1) & 1;
-  hipError_t e = run_flux(c, c->prim, s, 1);
+  hipError_t e = run_flux(c, h.P, s, 1, h.r1);
   if (e != hipSuccess) return e;
   UpdArgs a{};
-  a.in = c->ext;
-  a.out = c->mid_ext;
+  a.in = h.X;
+  a.out = h.Xm;
   a.flux = c->flux;
-  a.cells = c->mid_prim;
+  a.cells = h.Pm;
   a.dt_slot = c->words + cur;
   a.dt_hist = dt_hist;
   a.step = hist;
@@ -354,14 +375,14 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist) {
     k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c), derr(c, s, 3));
   });
   if (e != hipSuccess) return e;
-  e = run_flux(c, c->mid_prim, s, 4);
+  e = run_flux(c, h.Pm, s, 4, h.r2);
   if (e != hipSuccess) return e;
   UpdArgs b{};
-  b.in = c->mid_ext;
-  b.out = c->ext;
-  b.avg = c->ext;
+  b.in = h.Xm;
+  b.out = h.Xo;
+  b.avg = h.X;
   b.flux = c->flux;
-  b.cells = c->prim;
+  b.cells = h.Po;
   b.dt_slot = c->words + cur;
   b.partials = c->partials;
   b.step = hist;
@@ -1005,7 +1026,7 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
       HIPCHK(c, run_update(c, a, s, 3));
       HIPCHK(c, run_reduce_min(c, nxt));
     } else {
-      HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s));
+      HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s, ctx_bufs(c)));
     }
   }
   if (dt_out)
@@ -1038,17 +1059,18 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
     HIPCHK(c, get(&c->ckX, n * nck));
     HIPCHK(c, get(&c->finP, n));
     HIPCHK(c, get(&c->finX, n));
-    HIPCHK(c, get(&c->tP, n * ev));
-    HIPCHK(c, get(&c->tX, n * ev));
+    HIPCHK(c, get(&c->tP, n * (ev + 1)));
+    HIPCHK(c, get(&c->tX, n * (ev + 1)));
     HIPCHK(c, get(&c->tPm, n * ev));
     HIPCHK(c, get(&c->tXm, n * ev));
+    HIPCHK(c, get(&c->tR1, n * ev));
+    HIPCHK(c, get(&c->tR2, n * ev));
     HIPCHK(c, get(&c->tdt, static_cast<size_t>(ev)));
     HIPCHK(c, get(&c->gP, n));
     HIPCHK(c, get(&c->gX, n));
     HIPCHK(c, get(&c->ga, n));
     HIPCHK(c, get(&c->gb, n));
     HIPCHK(c, get(&c->gps, n));
-    HIPCHK(c, get(&c->gslope, 2 * n));
     HIPCHK(c, get(&c->lb, static_cast<size_t>(F)));
     HIPCHK(c, get(&c->rb, static_cast<size_t>(F)));
     HIPCHK(c, get(&c->emit, ne));
@@ -1074,7 +1096,7 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
       HIPCHK(c, hipMemcpyAsync(c->ckP + k, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(c, hipMemcpyAsync(c->ckX + k, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
-    HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s));
+    HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s, ctx_bufs(c)));
   }
   HIPCHK(c, hipMemcpyAsync(c->finP, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->finX, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -1091,34 +1113,28 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
 
 namespace {
 
-// Reverse of one Heun stage on the cells q (P_n or P_mid of step t of the
-// tape): seed c->ga = adjoint of the stage's output extensives.  mode 2
-// (second stage) leaves the adjoint of the mid extensives in c->ga; mode 1
-// (first stage) writes the step-start adjoints to c->gP and c->gX.
-hipError_t adj_stage(rg_ctx* c, const D4* q, const D4* X, const double* dtp, double* dtb,
-                     int mode, int step) {
+// Reverse of one Heun stage on the cells q with the taped slopes rec: seed
+// c->ga = adjoint of the stage's output extensives.  mode 2 (second stage)
+// leaves the adjoint of the mid extensives in c->ga; mode 1 (first stage)
+// writes the step-start adjoints to c->gP and c->gX and the per-block CFL
+// argmin of q.
+hipError_t adj_stage(rg_ctx* c, const D4* q, const Rec* rec, const D4* X, const double* dtp,
+                     double* dtb, int mode) {
   const int N = c->N, F = c->F;
-  hipError_t e = run_slopes(c, q, step, mode == 2 ? 4 : 1);
-  if (e != hipSuccess) return e;
-  e = launch(c, KID_ADJ_FLUX, [&] {
-    k_adj_flux<<<blocks_for(F), kBlock, 0, c->stream>>>(c->dm, q, c->rec, c->ga, dtp, c->prm.cfl,
+  hipError_t e = launch(c, KID_ADJ_FLUX, [&] {
+    k_adj_flux<<<blocks_for(F), kBlock, 0, c->stream>>>(c->dm, q, rec, c->ga, dtp, c->prm.cfl,
                                                          c->lb, c->rb, dtb, c->prm.gamma);
   });
   if (e != hipSuccess) return e;
-  e = launch(c, KID_ADJ_CELL, [&] {
-    k_adj_cellstate<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->am, c->lb, c->rb, c->gps,
-                                                              c->gslope);
-  });
-  if (e != hipSuccess) return e;
   e = launch(c, KID_ADJ_SLOPES, [&] {
-    k_adj_slopes<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, q, c->gslope, c->gps, c->emit,
-                                                           dparams(c));
+    k_adj_slopes<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->am, q, c->lb, c->rb, c->gps,
+                                                           c->emit, dparams(c));
   });
   if (e != hipSuccess) return e;
   return launch(c, KID_ADJ_CELL, [&] {
     k_adj_gather<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->am, c->gps, c->emit, mode, X,
-                                                           c->ga, c->gb, c->gP, c->gX,
-                                                           c->prm.gamma);
+                                                           c->ga, c->gb, c->gP, c->gX, q, c->vmin,
+                                                           c->imin, c->prm.gamma);
   });
 }
 
@@ -1141,47 +1157,39 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
   } else {
     HIPCHK(c, hipMemsetAsync(c->gX, 0, bytes, c->stream));
   }
+  PendingDt pd{nullptr, c->dtb, static_cast<int>(2 * nbF), c->vmin, c->imin,
+               static_cast<int>(nbN), c->prm.cfl};
   for (int seg = c->adj_nck - 1; seg >= 0; --seg) {
     const int k0 = seg * every, k1 = std::min(k0 + every, nsteps), L = k1 - k0;
-    // recompute the segment from its checkpoint, taping each step's states
-    HIPCHK(c, hipMemcpyAsync(c->prim, c->ckP + static_cast<size_t>(seg) * n, bytes,
+    // recompute the segment from its checkpoint into the tape: states
+    // tP/tX[0..L], mid states, both stages' slopes and the dts
+    HIPCHK(c, hipMemcpyAsync(c->tP, c->ckP + static_cast<size_t>(seg) * n, bytes,
                              hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->ext, c->ckX + static_cast<size_t>(seg) * n, bytes,
+    HIPCHK(c, hipMemcpyAsync(c->tX, c->ckX + static_cast<size_t>(seg) * n, bytes,
                              hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, run_cfl(c, c->prim, k0 & 1));
+    HIPCHK(c, run_cfl(c, c->tP, k0 & 1));
     for (int t = 0; t < L; ++t) {
       const size_t o = static_cast<size_t>(t) * n;
-      HIPCHK(c, hipMemcpyAsync(c->tP + o, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->tX + o, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(c, heun_step(c, k0 + t, c->tdt, t));
-      HIPCHK(c, hipMemcpyAsync(c->tPm + o, c->mid_prim, bytes, hipMemcpyDeviceToDevice,
-                               c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->tXm + o, c->mid_ext, bytes, hipMemcpyDeviceToDevice,
-                               c->stream));
+      const HeunBufs h{c->tP + o,  c->tX + o,     c->tPm + o,    c->tXm + o,
+                       c->tP + o + n, c->tX + o + n, c->tR1 + o, c->tR2 + o};
+      HIPCHK(c, heun_step(c, k0 + t, c->tdt, t, h));
     }
-    // reverse through the segment; c->ext holds X at the segment's end
+    // reverse through the segment
     for (int t = L - 1; t >= 0; --t) {
       const size_t o = static_cast<size_t>(t) * n;
-      const D4* Xn1 = (t + 1 < L) ? c->tX + o + n : c->ext;
+      // the dt adjoint of step k0+t+1 is pending except for the last step
+      pd.q = (k0 + t + 1 < nsteps) ? c->tP + o + n : nullptr;
       HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
-        k_adj_recover<<<nbN, kBlock, 0, c->stream>>>(c->dm, Xn1, c->gP, c->gX, c->ga, c->gb,
-                                                     c->prm.gamma);
+        k_adj_recover<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->tX + o + n, c->gP, c->gX, c->ga,
+                                                     c->gb, c->prm.gamma, pd);
       }));
-      HIPCHK(c, adj_stage(c, c->tPm + o, c->tXm + o, c->tdt + t, c->dtb, 2, k0 + t));
-      HIPCHK(c, adj_stage(c, c->tP + o, nullptr, c->tdt + t, c->dtb + nbF, 1, k0 + t));
-      HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
-        k_adj_argmin<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->tP + o, c->prm.gamma, c->vmin,
-                                                    c->imin);
-      }));
-      HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
-        k_adj_dt<<<1, kBlock, 0, c->stream>>>(c->dm, c->tP + o, c->dtb, static_cast<int>(2 * nbF),
-                                              c->vmin, c->imin, static_cast<int>(nbN),
-                                              c->prm.cfl, c->prm.gamma, c->gP);
-      }));
+      HIPCHK(c, adj_stage(c, c->tPm + o, c->tR2 + o, c->tXm + o, c->tdt + t, c->dtb, 2));
+      HIPCHK(c, adj_stage(c, c->tP + o, c->tR1 + o, nullptr, c->tdt + t, c->dtb + nbF, 1));
     }
   }
+  pd.q = c->ckP;  // the dt adjoint of step 0
   HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
-    k_adj_init<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->ckP, c->gX, c->gP, c->prm.gamma);
+    k_adj_init<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->ckP, c->gX, c->gP, c->prm.gamma, pd);
   }));
   {
     double* const dst[4] = {gcells0, gcells0 + n, gcells0 + 2 * n, gcells0 + 3 * n};

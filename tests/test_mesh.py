@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 from scipy.spatial import Delaunay
 
+import cases
 from rich_amd import mesh as rmesh
 
 
@@ -120,3 +121,21 @@ def test_kh_size():
     # interior faces ~3N (two lattice edges + one diagonal per cell) plus the
     # periodic boundary layer
     assert 3.0 < m.n_faces / m.n_cells < 3.05
+
+
+@pytest.mark.parametrize("n", [20, 64])
+def test_rigid_ghost_touches_only_its_original(n):
+    """RigidWallGenerator copies the real cell of a ghost's first boundary
+    face (RigidWallGenerator.cpp:5-37).  On these lattices every rigid ghost
+    borders exactly one face, shared with the cell it mirrors
+    (GetOriginalIndex), so that rule and the mirror rule coincide."""
+    m = cases.sedov_mesh(n, flip_seed=3)
+    N, G = m.n_cells, m.n_ghosts
+    count = np.zeros(G, int)
+    for f in range(m.n_faces):
+        a, b = m.face_first[f], m.face_second[f]
+        for g, r in ((a, b), (b, a)):
+            if g >= N:
+                count[g - N] += 1
+                assert r == m.ghost_orig[g - N]
+    assert np.all(count[count > 0] == 1)
