@@ -96,6 +96,89 @@ def cpu_baseline(mesh, cells, steps, threads):
     return c, e, dts, dt
 
 
+def adjoint_leg(args):
+    """Sedov inverse problem (static rigid box): timings of the checkpointed
+    forward and of the adjoint, and the adjoint's directional derivatives
+    against central differences (h = 1e-6, directions relative to the
+    parameters) and optionally against the forward-mode CPU oracle."""
+    import numpy as np
+    import torch
+    from rich_amd.inverse import SedovInverse
+
+    n, steps, every = args.adjoint_n, args.adjoint_steps, args.adjoint_every
+    P = SedovInverse(n=n, steps=steps, every=every)
+    s = P.solver
+    u0 = P.u0
+    E = torch.tensor(1.0, dtype=torch.float64)
+    c0 = P.cells(u0, E).numpy().copy()
+    w = np.random.default_rng(1).standard_normal((4, P.N))
+    plain, fck, bwd = [], [], []
+    for _ in range(3):
+        s.set_cells(c0)
+        s.init_extensives()
+        t = time.perf_counter()
+        s.advance(steps, integrator=2, record_dt=False)
+        plain.append(time.perf_counter() - t)
+        s.set_cells(c0)
+        s.init_extensives()
+        t = time.perf_counter()
+        s.adjoint_forward(steps, every, record_dt=False)
+        fck.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        s.adjoint_backward(w)
+        bwd.append(time.perf_counter() - t)
+    tf, tc, tb = (float(np.median(x)) for x in (plain, fck, bwd))
+    out = {"workload": f"Sedov {n}x{n} jittered lattice, rigid box [-1,1]^2, static mesh, "
+                       f"{steps} Heun steps, checkpoint every {every}, FP64",
+           "cells": P.N, "forward_ms": 1e3 * tf, "forward_checkpointed_ms": 1e3 * tc,
+           "backward_ms": 1e3 * tb, "backward_over_forward": tb / tf,
+           "forward_and_backward_over_forward": (tc + tb) / tf}
+    rng = np.random.default_rng(0)
+    dirs = [(torch.from_numpy(rng.standard_normal(P.N)), float(rng.standard_normal()))
+            for _ in range(args.adjoint_dirs)]
+    h = 1e-6
+    for key, r_ring in (("ring_0.30_as_specified", 0.3), ("ring_0.22_at_the_shock", 0.22)):
+        Q = P if r_ring == 0.3 else SedovInverse(n=n, steps=steps, every=every, r_ring=r_ring)
+        L, gu, gE = Q.gradient(u0, E)
+        errs, an_all = [], []
+        for xi, xe in dirs:
+            du, dE = xi * u0, xe * E
+            an = float((gu * du).sum()) + gE * float(dE)
+            fd = (Q.loss_value(u0 + h * du, E + h * dE) -
+                  Q.loss_value(u0 - h * du, E - h * dE)) / (2 * h)
+            errs.append(abs(fd - an) / max(abs(an), 1e-300))
+            an_all.append(an)
+        fdE = (Q.loss_value(u0, E + h) - Q.loss_value(u0, E - h)) / (2 * h)
+        rec = {"loss": L, "dL_dE": gE, "grad_u_norm": float(gu.norm()), "h": h,
+               "fd_directions": len(errs), "fd_rel_err_median": float(np.median(errs)),
+               "fd_rel_err_max": float(np.max(errs)),
+               "fd_within_1e-6": int(np.sum(np.array(errs) <= 1e-6)),
+               "E_only_fd": fdE, "E_only_fd_rel_err": abs(fdE - gE) / max(abs(gE), 1e-300)}
+        if args.adjoint_cpu_ref > 0:
+            import pyoracle as po
+            threads = min(16, os.cpu_count() or 1)
+            o = po.Oracle(Q.mesh, order=2, nthreads=threads)
+            K = min(args.adjoint_cpu_ref, len(dirs))
+            dc = np.zeros((K, 4, P.N))
+            for k in range(K):
+                xi, xe = dirs[k]
+                dc[k, 1] = (Q.gamma - 1.0) * (xi * u0 + xe * Q.w).numpy()
+            t = time.perf_counter()
+            cT, dcT, _, _, _ = o.run_tangent(c0, dc, steps)
+            ring = Q.ring.numpy()
+            resid = cT[0, ring] - Q.target.numpy()
+            ref = [float(np.sum(2.0 * resid * dcT[k, 0, ring])) for k in range(K)]
+            rel = [abs(an_all[k] - ref[k]) / max(abs(ref[k]), 1e-300) for k in range(K)]
+            rec["cpu_reference"] = {"directions": K, "rel_err_max": float(np.max(rel)),
+                                    "seconds": time.perf_counter() - t, "threads": threads,
+                                    "kind": "forward-mode oracle (dual numbers)"}
+        out[key] = rec
+        if Q is not P:
+            Q.close()
+    P.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +193,15 @@ def main():
                     help="time without per-kernel HIP events")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision run (FP32 reconstruction and HLLC)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-n", type=int, default=256)
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--adjoint-dirs", type=int, default=32,
+                    help="seeded random directions of the finite-difference check")
+    ap.add_argument("--adjoint-cpu-ref", type=int, default=8, metavar="K",
+                    help="also compare K directional derivatives with the forward-mode CPU "
+                         "oracle (dual numbers, OpenMP, 8 per pass); 0 = skip")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HIP runtime copy "
                          "kernel) and exit: the reference point for HBM byte counters")
@@ -237,6 +329,8 @@ def main():
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}
     if mixed:
         line["mixed"] = mixed
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(args)
     if cpu:
         line["cpu_baseline"] = cpu
         line["speedup_vs_cpu"] = value / cpu["value"]
