@@ -96,6 +96,70 @@ def cpu_baseline(mesh, cells, steps, threads):
     return c, e, dts, dt
 
 
+def ensemble_leg(args):
+    """256 independent Sedov runs of uneven size advanced together (one
+    hipGraph launch of persistent kernels per step, per-instance dt, launch
+    shape from the autotuner), against the same instances advanced one at a
+    time."""
+    import numpy as np
+    from rich_amd import ensemble
+    from rich_amd.gpu import RichGPU
+
+    t0 = time.perf_counter()
+    ms, cells, prm = ensemble.sedov_ensemble(K=args.ensemble_k)
+    u = ensemble.union(ms)
+    g = RichGPU(u.mesh, order=2, cfl=0.3)
+    g.set_cells(np.concatenate(cells, axis=1))
+    g.init_extensives()
+    g.set_instances(u.cell_off)
+    setup = time.perf_counter() - t0
+    table = g.ensemble_autotune(trial_steps=3)
+    best = min(table, key=lambda r: r[2])
+    steps = args.ensemble_steps
+    g.advance_ensemble(2, record_dt=False)  # warm-up
+    g.set_cells(np.concatenate(cells, axis=1))
+    g.init_extensives()
+    t = time.perf_counter()
+    dts = g.advance_ensemble(steps)
+    el = time.perf_counter() - t
+    final = u.split(g.get_cells())
+    g.close()
+    N = int(u.cell_off[-1])
+    # a sample advanced alone: the same bits, and the sequential rate
+    sample = list(range(0, len(ms), max(1, len(ms) // 8)))[:8]
+    same, t_alone, n_alone = 0, 0.0, 0
+    for k in sample:
+        a = RichGPU(ms[k], order=2, cfl=0.3)
+        a.set_cells(cells[k])
+        a.init_extensives()
+        a.advance(2, integrator=2, record_dt=False)
+        a.set_cells(cells[k])
+        a.init_extensives()
+        t = time.perf_counter()
+        adt = a.advance(steps, integrator=2)
+        t_alone += time.perf_counter() - t
+        n_alone += ms[k].n_cells
+        same += int(np.array_equal(a.get_cells(), final[k]) and np.array_equal(adt, dts[:, k]))
+        a.close()
+    value = N * steps / el / 1e6
+    seq_rate = n_alone * steps / t_alone / 1e6
+    return {"workload": f"{len(ms)} independent Sedov runs, square jittered lattices "
+                        f"{int(prm['n'].min())}^2..{int(prm['n'].max())}^2 cells in [-1,1]^2, "
+                        "rigid walls, E log-uniform [0.1,10], deposit radius 3 cell spacings, "
+                        "second order, Heun, static mesh, FP64, per-instance CFL dt",
+            "instances": len(ms), "cells_total": N, "faces_total": int(u.mesh.n_faces),
+            "steps": steps, "value": value, "unit": "Mcell-updates/s",
+            "ms_per_step": 1e3 * el / steps,
+            "launch": {"block": best[0], "blocks_per_cu": best[1], "graph_launches_per_step": 1,
+                       "kernels_per_step": 8},
+            "autotune_ms_per_step": [[b, p, round(t_, 4)] for b, p, t_ in table],
+            "one_at_a_time": {"instances": len(sample), "value": seq_rate,
+                              "batched_speedup": value / seq_rate},
+            "bit_identical_to_alone": f"{same}/{len(sample)}",
+            "dt_range": [float(dts[0].min()), float(dts[0].max())],
+            "setup_s": setup}
+
+
 def adjoint_leg(args):
     """Sedov inverse problem (static rigid box): timings of the checkpointed
     forward and of the adjoint, and the adjoint's directional derivatives
@@ -202,6 +266,9 @@ def main():
     ap.add_argument("--adjoint-cpu-ref", type=int, default=8, metavar="K",
                     help="also compare K directional derivatives with the forward-mode CPU "
                          "oracle (dual numbers, OpenMP, 8 per pass); 0 = skip")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ensemble-k", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HIP runtime copy "
                          "kernel) and exit: the reference point for HBM byte counters")
@@ -329,6 +396,8 @@ def main():
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}
     if mixed:
         line["mixed"] = mixed
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_leg(args)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(args)
     if cpu:

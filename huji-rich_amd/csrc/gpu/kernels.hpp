@@ -352,9 +352,9 @@ __global__ void __launch_bounds__(256) k_geo_ell(DMesh m, D2* __restrict__ eh, D
 
 // calc_slope (LinearGaussImproved.cpp:425-494) with the geometric half
 // taken from CellGeo / FaceRec.  Writes the cell's record.
-__global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
-                                                Rec* __restrict__ out, DParams prm, DErr err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict__ q,
+                                            Rec* __restrict__ out, const DParams& prm,
+                                            const DErr& err, int i) {
   if (i >= m.N) return;
   const int b = m.off[i], e = m.off[i + 1], deg = e - b;
   const int eb = ell_base(m, i);
@@ -490,6 +490,17 @@ __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ 
                sy.d,   sy.p,   sy.vx,   sy.vy,   g.cmx, g.cmy, 0,     0};
 }
 
+__global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
+                                                Rec* __restrict__ out, DParams prm, DErr err) {
+  slopes_cell(m, q, out, prm, err, blockIdx.x * blockDim.x + threadIdx.x);
+}
+// Persistent form: a fixed grid walks the cells in tiles of blockDim.x.
+__global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict__ q,
+                                                  Rec* __restrict__ out, DParams prm, DErr err) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.N; i += gridDim.x * blockDim.x)
+    slopes_cell(m, q, out, prm, err, i);
+}
+
 // ---------------------------------------------------------------- fluxes --
 // Face state on one side (LinearGaussImproved.cpp:571-590 for real cells,
 // :597-634 for ghosts: the ghost value plus the ghost gradient -- zero for
@@ -513,10 +524,9 @@ __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Re
   return interp(fetch(m, q, id), z, z, fx, fy, cg.x, cg.y);
 }
 
-__global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
-                                               const Rec* __restrict__ rec, D4* __restrict__ flux,
-                                               DParams prm, DErr err) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict__ q,
+                                           const Rec* __restrict__ rec, D4* __restrict__ flux,
+                                           const DParams& prm, const DErr& err, int f) {
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
   const Cell lf = side_state(m, q, rec, m.ff[f], fr.cx, fr.cy);
@@ -535,6 +545,19 @@ __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q
 }
 
 // SpatialReconstruction output res[f].first / .second
+__global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
+                                               const Rec* __restrict__ rec, D4* __restrict__ flux,
+                                               DParams prm, DErr err) {
+  flux2_face(m, q, rec, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x);
+}
+// Persistent form: a fixed grid walks the faces in tiles of blockDim.x.
+__global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__ q,
+                                                 const Rec* __restrict__ rec,
+                                                 D4* __restrict__ flux, DParams prm, DErr err) {
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < m.F; f += gridDim.x * blockDim.x)
+    flux2_face(m, q, rec, flux, prm, err, f);
+}
+
 __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restrict__ q,
                                                      const Rec* __restrict__ rec, D4* left,
                                                      D4* right) {
@@ -655,16 +678,13 @@ struct UpdArgs {
 // gather in ascending face order, ExternalForceContribution with ZeroForce
 // (hydrodynamics_2d.cpp:348-372), average_extensive (hdsim2d.cpp:311-320),
 // regular_update (simple_cell_updater.cpp:22-59), TimeStepCalculator.
-__global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm, DErr err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double dt = a.dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot))
-                              : a.dt;
-  if (i == 0) {
-    if (a.dt_hist) a.dt_hist[a.step] = dt;
-    if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
-  }
+// One cell with a given dt; returns its CFL inverse time step when
+// UPD_CFL is set (else +inf).
+__device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
+                                              const DParams& prm, const DErr& err, int i,
+                                              double dt) {
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
-  if (i < m.N) {
+  {
     const D4 x = a.in[i];
     double em = x.a, eE = x.b, ex = x.c, ey = x.d;
     if (a.flags & UPD_ZF_BEFORE) {
@@ -734,6 +754,19 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
       if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, m.width[i], Cell{d, p, vx, vy});
     }
   }
+  return cfl_v;
+}
+
+__global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm, DErr err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = a.dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot))
+                              : a.dt;
+  if (i == 0) {
+    if (a.dt_hist) a.dt_hist[a.step] = dt;
+    if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
+  }
+  double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  if (i < m.N) cfl_v = update_cell(m, a, prm, err, i, dt);
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
 }
 

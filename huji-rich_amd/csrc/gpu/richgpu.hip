@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "adjoint.hpp"
+#include "ensemble.hpp"
 #include "kernels_mixed.hpp"
 
 using namespace rg;
@@ -38,6 +39,7 @@ enum KernelId : int {
   KID_ADJ_FLUX,
   KID_ADJ_SLOPES,
   KID_ADJ_CELL,
+  KID_ENS,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -45,7 +47,7 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_init_ext",    "k_fill_u64", "k_geo",
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
-                                             "k_adj_slopes",  "k_adj_cell"};
+                                             "k_adj_slopes",  "k_adj_cell",     "k_ens"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -111,6 +113,13 @@ struct rg_ctx {
   D4 *lb = nullptr, *rb = nullptr, *emit = nullptr;
   double *dtb = nullptr, *vmin = nullptr;
   int* imin = nullptr;
+
+  // batched ensemble (rg_set_instances): per-instance dt, graph per step
+  int ens_K = 0, ens_cu = 0, ens_block = 256, ens_bpc = 4, ens_hist_cap = 0;
+  int *ens_inst = nullptr, *ens_coff = nullptr, *ens_step = nullptr;
+  unsigned long long *ens_dtA = nullptr, *ens_dtB = nullptr;
+  double *ens_inv = nullptr, *ens_hist = nullptr;
+  hipGraphExec_t ens_graph = nullptr;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -178,6 +187,15 @@ void free_mesh(rg_ctx* c) {
   c->dtb = c->vmin = nullptr;
   c->imin = nullptr;
   c->ext_init = false;
+  if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
+  c->ens_graph = nullptr;
+  if (c->ens_hist) (void)hipFree(c->ens_hist);
+  c->ens_hist = nullptr;
+  c->ens_hist_cap = 0;
+  c->ens_K = 0;
+  c->ens_inst = c->ens_coff = c->ens_step = nullptr;
+  c->ens_dtA = c->ens_dtB = nullptr;
+  c->ens_inv = nullptr;
   c->recM = nullptr;
   c->has_mesh = c->has_cells = c->has_ext = c->has_flux = false;
 }
@@ -392,6 +410,73 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   });
   if (e != hipSuccess) return e;
   return run_reduce_min(c, nxt);
+}
+
+Ens ens_args(rg_ctx* c) {
+  return Ens{c->ens_K,   c->ens_inst, c->ens_coff,     c->ens_dtA,  c->ens_dtB,
+             c->ens_inv, c->ens_hist, c->ens_step,     c->ens_hist_cap};
+}
+
+// The first dt of every instance from the current cells.
+hipError_t ens_seed(rg_ctx* c) {
+  const Ens e = ens_args(c);
+  const unsigned grid = static_cast<unsigned>(c->ens_cu * c->ens_bpc);
+  return launch(c, KID_ENS, [&] {
+    k_inv_ens_p<<<grid, c->ens_block, 0, c->stream>>>(c->dm, c->prim, e, c->prm.gamma);
+    k_ens_min<<<static_cast<unsigned>(std::min(e.K, 4096)), 256, 0, c->stream>>>(e);
+    k_ens_finish<<<1, 256, 0, c->stream>>>(e, c->prm.cfl, 0);
+  });
+}
+
+// The eight persistent launches of one Heun step of every instance
+// (hdsim::TimeAdvance2Heun, hdsim2d.cpp:323-449), each instance with its
+// own dt.  Everything that changes between steps is in device memory.
+hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
+  const Ens e = ens_args(c);
+  const DParams prm = dparams(c);
+  const unsigned grid = static_cast<unsigned>(c->ens_cu * bpc);
+  UpdArgs a{};
+  a.in = c->ext;
+  a.out = c->mid_ext;
+  a.flux = c->flux;
+  a.cells = c->mid_prim;
+  a.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_CELLS;
+  UpdArgs b{};
+  b.in = c->mid_ext;
+  b.out = c->ext;
+  b.avg = c->ext;
+  b.flux = c->flux;
+  b.cells = c->prim;
+  b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
+  const hipStream_t st = c->stream;
+  k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
+  k_flux2_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, c->flux, prm, derr(c, 0, 2));
+  k_update_ens_p<<<grid, block, 0, st>>>(c->dm, a, e, prm, derr(c, 0, 3));
+  k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, prm, derr(c, 0, 4));
+  k_flux2_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, c->flux, prm, derr(c, 0, 5));
+  k_update_ens_p<<<grid, block, 0, st>>>(c->dm, b, e, prm, derr(c, 0, 6));
+  k_ens_min<<<static_cast<unsigned>(std::min(e.K, 4096)), 256, 0, st>>>(e);
+  k_ens_finish<<<1, 256, 0, st>>>(e, c->prm.cfl, 1);
+  return hipGetLastError();
+}
+
+// Capture one step into c->ens_graph with the given launch shape.
+hipError_t ens_capture(rg_ctx* c, int block, int bpc) {
+  if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
+  c->ens_graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t e1 = ens_record_step(c, block, bpc);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(c->stream, &g);
+  if (e1 != hipSuccess) e = e1;
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->ens_graph, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e == hipSuccess) {
+    c->ens_block = block;
+    c->ens_bpc = bpc;
+  }
+  return e;
 }
 
 // hdsim::TimeAdvance2Heun with FP32 reconstruction and HLLC and FP64
@@ -1004,6 +1089,32 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
     c->dt_hist_cap = nsteps;
   }
   c->ext_init = false;
+  if (c->ens_K > 0) {
+    if (integrator != 2 || c->prm.order != 2 || c->prm.precision != RG_FP64 || c->dm.wx)
+      return RG_E_ARG;
+    if (nsteps > c->ens_hist_cap) {
+      if (c->ens_hist) (void)hipFree(c->ens_hist);
+      c->ens_hist = nullptr;
+      c->ens_hist_cap = 0;
+      HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->ens_hist),
+                          sizeof(double) * static_cast<size_t>(nsteps) * c->ens_K));
+      c->ens_hist_cap = nsteps;
+      if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);  // holds the old pointer
+      c->ens_graph = nullptr;
+    }
+    const bool prof = c->prof;
+    c->prof = false;  // no per-launch events inside a graph
+    HIPCHK(c, hipMemsetAsync(c->ens_step, 0, sizeof(int), c->stream));
+    HIPCHK(c, ens_seed(c));
+    if (!c->ens_graph) HIPCHK(c, ens_capture(c, c->ens_block, c->ens_bpc));
+    for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->ens_graph, c->stream));
+    c->prof = prof;
+    if (dt_out)
+      HIPCHK(c, hipMemcpyAsync(dt_out, c->ens_hist,
+                               sizeof(double) * static_cast<size_t>(nsteps) * c->ens_K,
+                               hipMemcpyDeviceToHost, c->stream));
+    return collect_error(c);
+  }
   if (c->prm.precision == RG_MIXED && integrator == 2 && c->prm.order == 2)
     return advance_mixed(c, nsteps, dt_out);
   // first dt: SimpleCFL on the current cells into slot 0
@@ -1199,6 +1310,123 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
   // leave the forward run's final state in place
   HIPCHK(c, hipMemcpyAsync(c->prim, c->finP, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->ext, c->finX, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return collect_error(c);
+}
+
+int rg_set_instances(rg_ctx* c, int32_t n_inst, const int32_t* cell_off) {
+  if (!c || !c->has_mesh || n_inst <= 0 || !cell_off || cell_off[0] != 0 ||
+      cell_off[n_inst] != c->N)
+    return RG_E_ARG;
+  for (int k = 0; k < n_inst; ++k)
+    if (cell_off[k + 1] <= cell_off[k]) return RG_E_ARG;
+  const int N = c->N, F = c->F, G = c->G;
+  std::vector<int> inst(static_cast<size_t>(N));
+  for (int k = 0; k < n_inst; ++k)
+    for (int i = cell_off[k]; i < cell_off[k + 1]; ++i) inst[i] = k;
+  // every face and ghost must belong to one instance
+  std::vector<int> ff(static_cast<size_t>(F)), fs(static_cast<size_t>(F)),
+      go(static_cast<size_t>(std::max(G, 1)));
+  HIPCHK(c, hipMemcpy(ff.data(), c->dm.ff, sizeof(int) * F, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(fs.data(), c->dm.fs, sizeof(int) * F, hipMemcpyDeviceToHost));
+  if (G) HIPCHK(c, hipMemcpy(go.data(), c->dm.gorig, sizeof(int) * G, hipMemcpyDeviceToHost));
+  auto owner = [&](int id) { return inst[id < N ? id : go[id - N]]; };
+  for (int f = 0; f < F; ++f)
+    if (owner(ff[f]) != owner(fs[f])) return RG_E_ARG;
+  int dev = 0, cu = 0;
+  HIPCHK(c, hipGetDevice(&dev));
+  HIPCHK(c, hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPCHK(c, upload(c, &c->ens_inst, inst.data(), inst.size()));
+  HIPCHK(c, upload(c, &c->ens_coff, cell_off, static_cast<size_t>(n_inst) + 1));
+  HIPCHK(c, dalloc(c, &c->ens_dtA, n_inst));
+  HIPCHK(c, dalloc(c, &c->ens_dtB, n_inst));
+  HIPCHK(c, dalloc(c, &c->ens_inv, N));
+  HIPCHK(c, dalloc(c, &c->ens_step, 1));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
+  c->ens_graph = nullptr;
+  c->ens_K = n_inst;
+  c->ens_cu = cu;
+  return RG_OK;
+}
+
+int rg_ensemble_config(rg_ctx* c, int32_t block, int32_t blocks_per_cu) {
+  if (!c || c->ens_K <= 0 || (block != 64 && block != 128 && block != 256) || blocks_per_cu < 1 ||
+      blocks_per_cu > 64)
+    return RG_E_ARG;
+  c->ens_block = block;
+  c->ens_bpc = blocks_per_cu;
+  if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
+  c->ens_graph = nullptr;
+  return RG_OK;
+}
+
+int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t table_rows,
+                         int32_t* rows_out) {
+  if (!ready(c, true, true) || c->ens_K <= 0 || trial_steps <= 0) return RG_E_ARG;
+  const size_t bytes = sizeof(D4) * static_cast<size_t>(c->N);
+  D4 *saveP = nullptr, *saveX = nullptr;
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&saveP), bytes));
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&saveX), bytes));
+  HIPCHK(c, hipMemcpyAsync(saveP, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(saveX, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (trial_steps + 1 > c->ens_hist_cap) {
+    if (c->ens_hist) (void)hipFree(c->ens_hist);
+    c->ens_hist = nullptr;
+    c->ens_hist_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->ens_hist),
+                        sizeof(double) * static_cast<size_t>(trial_steps + 1) * c->ens_K));
+    c->ens_hist_cap = trial_steps + 1;
+  }
+  const bool prof = c->prof;
+  c->prof = false;
+  hipEvent_t t0, t1;
+  HIPCHK(c, hipEventCreate(&t0));
+  HIPCHK(c, hipEventCreate(&t1));
+  const int blocks[3] = {64, 128, 256};
+  const int bpcs[5] = {1, 2, 4, 8, 16};
+  double best = 1e300;
+  int best_block = 256, best_bpc = 4, rows = 0;
+  for (int bi = 0; bi < 3; ++bi)
+    for (int pi = 0; pi < 5; ++pi) {
+      const int block = blocks[bi], bpc = bpcs[pi];
+      if (block * bpc > 2048) continue;  // at most 32 waves per CU
+      HIPCHK(c, hipMemcpyAsync(c->prim, saveP, bytes, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->ext, saveX, bytes, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipMemsetAsync(c->ens_step, 0, sizeof(int), c->stream));
+      c->ens_block = block;
+      c->ens_bpc = bpc;
+      HIPCHK(c, ens_seed(c));
+      HIPCHK(c, ens_capture(c, block, bpc));
+      HIPCHK(c, hipGraphLaunch(c->ens_graph, c->stream));  // warm-up step
+      HIPCHK(c, hipEventRecord(t0, c->stream));
+      for (int s = 0; s < trial_steps; ++s) HIPCHK(c, hipGraphLaunch(c->ens_graph, c->stream));
+      HIPCHK(c, hipEventRecord(t1, c->stream));
+      HIPCHK(c, hipEventSynchronize(t1));
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+      const double per = ms / trial_steps;
+      if (table && rows < table_rows) {
+        table[3 * rows] = block;
+        table[3 * rows + 1] = bpc;
+        table[3 * rows + 2] = per;
+      }
+      ++rows;
+      if (per < best) {
+        best = per;
+        best_block = block;
+        best_bpc = bpc;
+      }
+    }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  HIPCHK(c, hipMemcpyAsync(c->prim, saveP, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ext, saveX, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(saveP);
+  (void)hipFree(saveX);
+  c->prof = prof;
+  HIPCHK(c, ens_capture(c, best_block, best_bpc));
+  if (rows_out) *rows_out = rows;
   return collect_error(c);
 }
 

@@ -71,6 +71,10 @@ def lib():
     L.rg_update_cells.argtypes = [vp]
     L.rg_advance.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
     L.rg_adjoint_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.rg_set_instances.argtypes = [vp, ctypes.c_int32, vp]
+    L.rg_ensemble_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    L.rg_ensemble_autotune.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32,
+                                       P(ctypes.c_int32)]
     L.rg_adjoint_backward.argtypes = [vp, vp, vp, vp]
     L.rg_last_error.argtypes = [vp, P(Error)]
     L.rg_profile_enable.argtypes = [vp, ctypes.c_int32]
@@ -220,6 +224,30 @@ class RichGPU:
         out = np.empty((4, self.N))
         self._check(self._lib.rg_adjoint_backward(self._h, _p(g), _p(x), _p(out)))
         return out
+
+    def set_instances(self, cell_off):
+        """Ensemble mode: the mesh is a disjoint union; cell_off (K+1,)."""
+        self._inst_off = np.ascontiguousarray(cell_off, dtype=np.int32)
+        self.K = len(self._inst_off) - 1
+        self._check(self._lib.rg_set_instances(self._h, self.K, _p(self._inst_off)))
+
+    def ensemble_config(self, block, blocks_per_cu):
+        self._check(self._lib.rg_ensemble_config(self._h, block, blocks_per_cu))
+
+    def ensemble_autotune(self, trial_steps=5):
+        """[(block, blocks_per_cu, ms per step)] of every launch shape tried;
+        the fastest is kept."""
+        table = np.zeros((32, 3))
+        rows = ctypes.c_int32()
+        self._check(self._lib.rg_ensemble_autotune(self._h, trial_steps, _p(table), 32,
+                                                   ctypes.byref(rows)))
+        return [(int(b), int(p), float(ms)) for b, p, ms in table[:rows.value]]
+
+    def advance_ensemble(self, nsteps, record_dt=True):
+        """rg_advance in ensemble mode; dts (nsteps, K)."""
+        dts = np.empty((max(nsteps, 1), self.K)) if record_dt else None
+        self._check(self._lib.rg_advance(self._h, 2, nsteps, _p(dts)))
+        return dts[:nsteps] if record_dt else None
 
     def set_stream(self, stream_ptr):
         self._check(self._lib.rg_set_stream(self._h, ctypes.c_void_p(stream_ptr)))

@@ -166,6 +166,25 @@ int rg_adjoint_forward(rg_ctx* ctx, int32_t nsteps, int32_t every, double* dt_ou
  * forward run is left in the context. */
 int rg_adjoint_backward(rg_ctx* ctx, const double* gcells, const double* gext, double* gcells0);
 
+/* batched ensemble -------------------------------------------------- */
+/* No counterpart in the reference (it runs one hdsim per simulation): for
+ * callers that advance many independent simulations together (parameter
+ * scans, ensembles).  The context's mesh is the disjoint union of n_inst
+ * meshes: the cells of instance k are cell_off[k]..cell_off[k+1]-1 and
+ * every face and ghost belongs to one instance (checked; else RG_E_ARG).
+ * rg_advance then gives each instance its own SimpleCFL dt (dt_out is
+ * [nsteps][n_inst]) and runs each step as one hipGraph launch of persistent
+ * kernels; integrator 2, order 2, RG_FP64, zero face velocities. */
+int rg_set_instances(rg_ctx* ctx, int32_t n_inst, const int32_t* cell_off);
+/* Launch shape of the persistent kernels: block (face/cell tile) 64, 128
+ * or 256 threads and blocks_per_cu resident blocks per compute unit. */
+int rg_ensemble_config(rg_ctx* ctx, int32_t block, int32_t blocks_per_cu);
+/* Times trial_steps steps for every launch shape, keeps the fastest and
+ * restores the state; table[3*r..] = (block, blocks_per_cu, ms per step)
+ * for up to table_rows rows; *rows_out = rows measured. */
+int rg_ensemble_autotune(rg_ctx* ctx, int32_t trial_steps, double* table, int32_t table_rows,
+                         int32_t* rows_out);
+
 /* diagnostics -------------------------------------------------------- */
 int rg_last_error(rg_ctx* ctx, rg_error* err);
 /* Per-kernel HIP-event timing of every launch on the context's stream. */
