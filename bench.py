@@ -96,6 +96,11 @@ def cpu_baseline(mesh, cells, steps, threads):
     return c, e, dts, dt
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def ensemble_leg(args):
     """256 independent Sedov runs of uneven size advanced together (one
     hipGraph launch of persistent kernels per step, per-instance dt, launch
@@ -105,6 +110,7 @@ def ensemble_leg(args):
     from rich_amd import ensemble
     from rich_amd.gpu import RichGPU
 
+    log(f"ensemble: building {args.ensemble_k} instances")
     t0 = time.perf_counter()
     ms, cells, prm = ensemble.sedov_ensemble(K=args.ensemble_k)
     u = ensemble.union(ms)
@@ -113,6 +119,7 @@ def ensemble_leg(args):
     g.init_extensives()
     g.set_instances(u.cell_off)
     setup = time.perf_counter() - t0
+    log(f"ensemble: {int(u.cell_off[-1])} cells set up in {setup:.1f} s; autotuning")
     table = g.ensemble_autotune(trial_steps=3)
     best = min(table, key=lambda r: r[2])
     steps = args.ensemble_steps
@@ -170,6 +177,7 @@ def adjoint_leg(args):
     from rich_amd.inverse import SedovInverse
 
     n, steps, every = args.adjoint_n, args.adjoint_steps, args.adjoint_every
+    log(f"adjoint: Sedov {n}x{n}, {steps} steps")
     P = SedovInverse(n=n, steps=steps, every=every)
     s = P.solver
     u0 = P.u0
@@ -218,7 +226,9 @@ def adjoint_leg(args):
                "fd_rel_err_max": float(np.max(errs)),
                "fd_within_1e-6": int(np.sum(np.array(errs) <= 1e-6)),
                "E_only_fd": fdE, "E_only_fd_rel_err": abs(fdE - gE) / max(abs(gE), 1e-300)}
+        log(f"adjoint: {key} finite differences done")
         if args.adjoint_cpu_ref > 0:
+            log(f"adjoint: {key} forward-mode CPU reference")
             import pyoracle as po
             threads = min(16, os.cpu_count() or 1)
             o = po.Oracle(Q.mesh, order=2, nthreads=threads)
@@ -307,6 +317,7 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         g.advance(args.cpu_steps, integrator=2)
         gc = g.get_cells()
+        log(f"cpu baseline: {args.cpu_steps} steps on {threads} threads")
         c_ref, e_ref, dts_ref, cpu_s = cpu_baseline(mesh, cells, args.cpu_steps, threads)
         err, same = compare(gc, c_ref)
         parity = {"steps": args.cpu_steps, "max_scaled_err": err, "bit_identical": same}
@@ -318,6 +329,7 @@ def main():
         g.set_cells(cells)
         g.init_extensives()
 
+    log(f"fp64: {args.warmup} warm-up + {args.steps} timed steps")
     g.advance(args.warmup, integrator=2, record_dt=False)
     if not args.no_profile:
         g.profile(True)
@@ -336,6 +348,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         from rich_amd.gpu import RG_MIXED
+        log("mixed precision run")
         gm = RichGPU(mesh, order=2, cfl=0.3, precision=RG_MIXED)
         gm.set_cells(cells)
         gm.init_extensives()
