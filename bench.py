@@ -54,7 +54,7 @@ def measured_traffic(kernel, nx, ny):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if
     there is no measurement for it."""
-    path = os.path.join(ROOT, "profiles", f"kh{nx}x{ny}_fp64_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"kh{nx}x{ny}_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     text = open(path).read()
