@@ -418,9 +418,16 @@ Ens ens_args(rg_ctx* c) {
 }
 
 // The first dt of every instance from the current cells.
+// Grid of a launch over n items: bpc resident blocks per CU walking the
+// items (persistent), or with bpc = 0 one thread per item.
+unsigned ens_grid(const rg_ctx* c, int n, int block, int bpc) {
+  return bpc > 0 ? static_cast<unsigned>(c->ens_cu * bpc)
+                 : static_cast<unsigned>((n + block - 1) / block);
+}
+
 hipError_t ens_seed(rg_ctx* c) {
   const Ens e = ens_args(c);
-  const unsigned grid = static_cast<unsigned>(c->ens_cu * c->ens_bpc);
+  const unsigned grid = ens_grid(c, c->N, c->ens_block, c->ens_bpc);
   return launch(c, KID_ENS, [&] {
     k_inv_ens_p<<<grid, c->ens_block, 0, c->stream>>>(c->dm, c->prim, e, c->prm.gamma);
     k_ens_min<<<static_cast<unsigned>(std::min(e.K, 4096)), 256, 0, c->stream>>>(e);
@@ -434,7 +441,7 @@ hipError_t ens_seed(rg_ctx* c) {
 hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
   const Ens e = ens_args(c);
   const DParams prm = dparams(c);
-  const unsigned grid = static_cast<unsigned>(c->ens_cu * bpc);
+  const unsigned grid = ens_grid(c, c->N, block, bpc), gridF = ens_grid(c, c->F, block, bpc);
   UpdArgs a{};
   a.in = c->ext;
   a.out = c->mid_ext;
@@ -450,10 +457,10 @@ hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
   b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   const hipStream_t st = c->stream;
   k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
-  k_flux2_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, c->flux, prm, derr(c, 0, 2));
+  k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->prim, c->rec, c->flux, prm, derr(c, 0, 2));
   k_update_ens_p<<<grid, block, 0, st>>>(c->dm, a, e, prm, derr(c, 0, 3));
   k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, prm, derr(c, 0, 4));
-  k_flux2_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, c->flux, prm, derr(c, 0, 5));
+  k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->mid_prim, c->rec, c->flux, prm, derr(c, 0, 5));
   k_update_ens_p<<<grid, block, 0, st>>>(c->dm, b, e, prm, derr(c, 0, 6));
   k_ens_min<<<static_cast<unsigned>(std::min(e.K, 4096)), 256, 0, st>>>(e);
   k_ens_finish<<<1, 256, 0, st>>>(e, c->prm.cfl, 1);
@@ -1350,7 +1357,7 @@ int rg_set_instances(rg_ctx* c, int32_t n_inst, const int32_t* cell_off) {
 }
 
 int rg_ensemble_config(rg_ctx* c, int32_t block, int32_t blocks_per_cu) {
-  if (!c || c->ens_K <= 0 || (block != 64 && block != 128 && block != 256) || blocks_per_cu < 1 ||
+  if (!c || c->ens_K <= 0 || (block != 64 && block != 128 && block != 256) || blocks_per_cu < 0 ||
       blocks_per_cu > 64)
     return RG_E_ARG;
   c->ens_block = block;
@@ -1383,11 +1390,11 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
   HIPCHK(c, hipEventCreate(&t0));
   HIPCHK(c, hipEventCreate(&t1));
   const int blocks[3] = {64, 128, 256};
-  const int bpcs[5] = {1, 2, 4, 8, 16};
+  const int bpcs[6] = {1, 2, 4, 8, 16, 0};  // 0: one thread per item
   double best = 1e300;
   int best_block = 256, best_bpc = 4, rows = 0;
   for (int bi = 0; bi < 3; ++bi)
-    for (int pi = 0; pi < 5; ++pi) {
+    for (int pi = 0; pi < 6; ++pi) {
       const int block = blocks[bi], bpc = bpcs[pi];
       if (block * bpc > 2048) continue;  // at most 32 waves per CU
       HIPCHK(c, hipMemcpyAsync(c->prim, saveP, bytes, hipMemcpyDeviceToDevice, c->stream));

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <unordered_map>
 #include <vector>
 #include <algorithm>
 
@@ -84,6 +85,120 @@ struct Sites {
   std::vector<int32_t> id;  // (nx+2)*(ny+2)
   int32_t& at(int i, int j) { return id[static_cast<size_t>(j + 1) * (nx + 2) + (i + 1)]; }
 };
+
+}  // namespace
+
+namespace {
+
+struct EdgeRef {
+  int32_t tlo, thi;  // triangles on both sides
+  int32_t first, second;
+  int32_t k;  // edge slot in tlo
+};
+
+// Faces, CSR and cell geometry from the Delaunay triangles and their
+// shared edges (steps shared by both builders); m holds N, G, bc, the points
+// and ghost_orig.
+void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>& edges,
+                 int flip_seed, const std::vector<int>& ghost_wall, double x0, double x1,
+                 double y0, double y1) {
+  const int32_t N = m->N, G = m->G;
+  std::sort(edges.begin(), edges.end(), [](const EdgeRef& a, const EdgeRef& b) {
+    return a.tlo != b.tlo ? a.tlo < b.tlo : a.k < b.k;
+  });
+  // 6. faces, dropping near-zero-length ones (VoronoiMesh.cpp:287-288)
+  uint64_t fs = static_cast<uint64_t>(flip_seed) * 0x9E3779B97F4A7C15ull + 12345u;
+  std::vector<std::vector<int32_t>> cell_faces(static_cast<size_t>(N));
+  for (auto& cf : cell_faces) cf.reserve(8);
+  for (const EdgeRef& e : edges) {
+    const Tri &tl = tris[e.tlo], &th = tris[e.thi];
+    const double lx = th.cc.x - tl.cc.x, ly = th.cc.y - tl.cc.y;
+    const double len = std::sqrt(lx * lx + ly * ly);
+    if (!(len > 1e-8 * std::sqrt(tl.r * th.r))) continue;
+    int32_t first = e.first, second = e.second;
+    if (flip_seed != 0 && (splitmix64(fs) & 1u)) std::swap(first, second);
+    const int32_t f = static_cast<int32_t>(m->ff.size());
+    m->ff.push_back(first);
+    m->fs.push_back(second);
+    m->v1x.push_back(tl.cc.x);
+    m->v1y.push_back(tl.cc.y);
+    m->v2x.push_back(th.cc.x);
+    m->v2y.push_back(th.cc.y);
+    if (first < N) cell_faces[first].push_back(f);
+    if (second < N) cell_faces[second].push_back(f);
+  }
+  m->F = static_cast<int32_t>(m->ff.size());
+  m->off.resize(static_cast<size_t>(N) + 1);
+  m->off[0] = 0;
+  for (int32_t c = 0; c < N; ++c)
+    m->off[c + 1] = m->off[c] + static_cast<int32_t>(cell_faces[c].size());
+  m->csr.resize(static_cast<size_t>(m->off[N]));
+  for (int32_t c = 0; c < N; ++c)
+    std::copy(cell_faces[c].begin(), cell_faces[c].end(), m->csr.begin() + m->off[c]);
+
+  // 7. cell geometry, in each cell's face order
+  m->vol.resize(N);
+  m->tvol.resize(N);
+  m->cmx.resize(static_cast<size_t>(N) + G);
+  m->cmy.resize(static_cast<size_t>(N) + G);
+  for (int32_t c = 0; c < N; ++c) {
+    const int32_t b = m->off[c], e = m->off[c + 1];
+    // Tessellation::GetVolume: fan about the generating point
+    double area = 0;
+    for (int32_t k = b; k < e; ++k) {
+      const int32_t f = m->csr[k];
+      const double p1x = m->v1x[f] - m->px[c], p1y = m->v1y[f] - m->py[c];
+      const double p2x = m->v2x[f] - m->px[c], p2y = m->v2y[f] - m->py[c];
+      area += 0.5 * std::fabs(p1x * p2y + p1y * (-p2x));
+    }
+    m->tvol[c] = area;
+    // SlabSymmetry::calcVolume and CalcCellCM: fan about the first vertex
+    // of the first face
+    double vol = 0, a2 = 0, pcx = 0, pcy = 0;
+    if (e > b) {
+      const int32_t f0 = m->csr[b];
+      const double ax = m->v1x[f0], ay = m->v1y[f0];
+      for (int32_t k = b + 1; k < e; ++k) {
+        const int32_t f = m->csr[k];
+        const double q1x = m->v1x[f] - ax, q1y = m->v1y[f] - ay;
+        const double q2x = m->v2x[f] - ax, q2y = m->v2y[f] - ay;
+        vol += 0.5 * std::fabs(q1x * q2y + q1y * (-q2x));
+        const double at = 0.5 * std::fabs(q1x * q2y + q1y * (-q2x));
+        a2 += at;
+        const double s = at / 3.;
+        pcx += ((ax + m->v1x[f]) + m->v2x[f]) * s;
+        pcy += ((ay + m->v1y[f]) + m->v2y[f]) * s;
+      }
+    }
+    m->vol[c] = vol;
+    m->cmx[c] = pcx / a2;
+    m->cmy[c] = pcy / a2;
+  }
+  for (int32_t g = 0; g < G; ++g) {
+    const int32_t o = m->gorig[g];
+    const int32_t id = N + g;
+    if (m->bc == RM_BC_RIGID) {
+      const int w = ghost_wall[g];
+      const double cx = m->cmx[o], cy = m->cmy[o];
+      if (w == 0) {
+        m->cmx[id] = cx + 2 * (x1 - cx);
+        m->cmy[id] = cy + 0.0;
+      } else if (w == 1) {
+        m->cmx[id] = cx + 0.0;
+        m->cmy[id] = cy + 2 * (y1 - cy);
+      } else if (w == 2) {
+        m->cmx[id] = cx + 2 * (x0 - cx);
+        m->cmy[id] = cy + 0.0;
+      } else {
+        m->cmx[id] = cx + 0.0;
+        m->cmy[id] = cy + 2 * (y0 - cy);
+      }
+    } else {
+      m->cmx[id] = m->cmx[o] + (m->px[id] - m->px[o]);
+      m->cmy[id] = m->cmy[o] + (m->py[id] - m->py[o]);
+    }
+  }
+}
 
 }  // namespace
 
@@ -263,11 +378,6 @@ extern "C" rm_mesh* rm_lattice_build(const rm_lattice_params* prm) {
   }
   // 5. Delaunay edges -> Voronoi faces.  Each edge is visited from its
   //    lower-numbered triangle, in triangle order and CCW edge order.
-  struct EdgeRef {
-    int32_t tlo, thi;  // triangles on both sides
-    int32_t first, second;
-    int32_t k;  // edge slot in tlo
-  };
   std::vector<EdgeRef> edges;
   edges.reserve(tris.size() * 2);
   auto square_of = [&](int i, int j) -> int32_t {
@@ -314,101 +424,100 @@ extern "C" rm_mesh* rm_lattice_build(const rm_lattice_params* prm) {
         if (tris[dg.t0].p[a] == tris[dg.t1].p[b] && n < 2) shared[n++] = tris[dg.t0].p[a];
     if (n == 2) consider(dg.t0, dg.t1, shared[0], shared[1]);
   }
-  std::sort(edges.begin(), edges.end(), [](const EdgeRef& a, const EdgeRef& b) {
-    return a.tlo != b.tlo ? a.tlo < b.tlo : a.k < b.k;
-  });
-  // 6. faces, dropping near-zero-length ones (VoronoiMesh.cpp:287-288)
-  uint64_t fs = static_cast<uint64_t>(prm->flip_seed) * 0x9E3779B97F4A7C15ull + 12345u;
-  std::vector<std::vector<int32_t>> cell_faces(static_cast<size_t>(N));
-  for (auto& cf : cell_faces) cf.reserve(8);
-  for (const EdgeRef& e : edges) {
-    const Tri &tl = tris[e.tlo], &th = tris[e.thi];
-    const double lx = th.cc.x - tl.cc.x, ly = th.cc.y - tl.cc.y;
-    const double len = std::sqrt(lx * lx + ly * ly);
-    if (!(len > 1e-8 * std::sqrt(tl.r * th.r))) continue;
-    int32_t first = e.first, second = e.second;
-    if (prm->flip_seed != 0 && (splitmix64(fs) & 1u)) std::swap(first, second);
-    const int32_t f = static_cast<int32_t>(m->ff.size());
-    m->ff.push_back(first);
-    m->fs.push_back(second);
-    m->v1x.push_back(tl.cc.x);
-    m->v1y.push_back(tl.cc.y);
-    m->v2x.push_back(th.cc.x);
-    m->v2y.push_back(th.cc.y);
-    if (first < N) cell_faces[first].push_back(f);
-    if (second < N) cell_faces[second].push_back(f);
-  }
-  m->F = static_cast<int32_t>(m->ff.size());
-  m->off.resize(static_cast<size_t>(N) + 1);
-  m->off[0] = 0;
-  for (int32_t c = 0; c < N; ++c)
-    m->off[c + 1] = m->off[c] + static_cast<int32_t>(cell_faces[c].size());
-  m->csr.resize(static_cast<size_t>(m->off[N]));
-  for (int32_t c = 0; c < N; ++c)
-    std::copy(cell_faces[c].begin(), cell_faces[c].end(), m->csr.begin() + m->off[c]);
+  finish_mesh(m, tris, edges, prm->flip_seed, ghost_wall, prm->x0, prm->x1, prm->y0, prm->y1);
+  return m;
+}
 
-  // 7. cell geometry, in each cell's face order
-  m->vol.resize(N);
-  m->tvol.resize(N);
-  m->cmx.resize(static_cast<size_t>(N) + G);
-  m->cmy.resize(static_cast<size_t>(N) + G);
-  for (int32_t c = 0; c < N; ++c) {
-    const int32_t b = m->off[c], e = m->off[c + 1];
-    // Tessellation::GetVolume: fan about the generating point
-    double area = 0;
-    for (int32_t k = b; k < e; ++k) {
-      const int32_t f = m->csr[k];
-      const double p1x = m->v1x[f] - m->px[c], p1y = m->v1y[f] - m->py[c];
-      const double p2x = m->v2x[f] - m->px[c], p2y = m->v2y[f] - m->py[c];
-      area += 0.5 * std::fabs(p1x * p2y + p1y * (-p2x));
-    }
-    m->tvol[c] = area;
-    // SlabSymmetry::calcVolume and CalcCellCM: fan about the first vertex
-    // of the first face
-    double vol = 0, a2 = 0, pcx = 0, pcy = 0;
-    if (e > b) {
-      const int32_t f0 = m->csr[b];
-      const double ax = m->v1x[f0], ay = m->v1y[f0];
-      for (int32_t k = b + 1; k < e; ++k) {
-        const int32_t f = m->csr[k];
-        const double q1x = m->v1x[f] - ax, q1y = m->v1y[f] - ay;
-        const double q2x = m->v2x[f] - ax, q2y = m->v2y[f] - ay;
-        vol += 0.5 * std::fabs(q1x * q2y + q1y * (-q2x));
-        const double at = 0.5 * std::fabs(q1x * q2y + q1y * (-q2x));
-        a2 += at;
-        const double s = at / 3.;
-        pcx += ((ax + m->v1x[f]) + m->v2x[f]) * s;
-        pcy += ((ay + m->v1y[f]) + m->v2y[f]) * s;
-      }
-    }
-    m->vol[c] = vol;
-    m->cmx[c] = pcx / a2;
-    m->cmy[c] = pcy / a2;
-  }
+extern "C" rm_mesh* rm_points_build(const rm_points_params* prm) {
+  if (!prm || prm->n_real < 3 || prm->n_ghost < 0 || prm->n_tri < 1 || !prm->x || !prm->y ||
+      !prm->tri || (prm->n_ghost > 0 && (!prm->ghost_orig || !prm->ghost_wall)) ||
+      !(prm->x1 > prm->x0) || !(prm->y1 > prm->y0))
+    return nullptr;
+  const int32_t N = prm->n_real, G = prm->n_ghost, NP = N + G;
+  rm_mesh* m = new rm_mesh();
+  m->N = N;
+  m->G = G;
+  m->bc = RM_BC_RIGID;
+  m->px.assign(prm->x, prm->x + NP);
+  m->py.assign(prm->y, prm->y + NP);
+  std::vector<int> ghost_wall(static_cast<size_t>(G));
   for (int32_t g = 0; g < G; ++g) {
-    const int32_t o = m->gorig[g];
-    const int32_t id = N + g;
-    if (prm->bc == RM_BC_RIGID) {
-      const int w = ghost_wall[g];
-      const double cx = m->cmx[o], cy = m->cmy[o];
-      if (w == 0) {
-        m->cmx[id] = cx + 2 * (prm->x1 - cx);
-        m->cmy[id] = cy + 0.0;
-      } else if (w == 1) {
-        m->cmx[id] = cx + 0.0;
-        m->cmy[id] = cy + 2 * (prm->y1 - cy);
-      } else if (w == 2) {
-        m->cmx[id] = cx + 2 * (prm->x0 - cx);
-        m->cmy[id] = cy + 0.0;
-      } else {
-        m->cmx[id] = cx + 0.0;
-        m->cmy[id] = cy + 2 * (prm->y0 - cy);
-      }
-    } else {
-      m->cmx[id] = m->cmx[o] + (m->px[id] - m->px[o]);
-      m->cmy[id] = m->cmy[o] + (m->py[id] - m->py[o]);
+    if (prm->ghost_orig[g] < 0 || prm->ghost_orig[g] >= N || prm->ghost_wall[g] < 0 ||
+        prm->ghost_wall[g] > 3) {
+      delete m;
+      return nullptr;
     }
+    m->gorig.push_back(prm->ghost_orig[g]);
+    ghost_wall[g] = prm->ghost_wall[g];
   }
+  auto P = [&](int32_t id) { return V2{m->px[id], m->py[id]}; };
+  // triangles counter-clockwise, numbered in Hilbert order of their centroids
+  double bx0 = m->px[0], bx1 = bx0, by0 = m->py[0], by1 = by0;
+  for (int32_t i = 1; i < NP; ++i) {
+    bx0 = std::min(bx0, m->px[i]);
+    bx1 = std::max(bx1, m->px[i]);
+    by0 = std::min(by0, m->py[i]);
+    by1 = std::max(by1, m->py[i]);
+  }
+  const uint64_t hn = 1u << 16;
+  std::vector<std::pair<uint64_t, int32_t>> order(static_cast<size_t>(prm->n_tri));
+  for (int32_t t = 0; t < prm->n_tri; ++t) {
+    const int32_t* q = prm->tri + 3 * static_cast<size_t>(t);
+    for (int k = 0; k < 3; ++k)
+      if (q[k] < 0 || q[k] >= NP) {
+        delete m;
+        return nullptr;
+      }
+    const double cx = (m->px[q[0]] + m->px[q[1]] + m->px[q[2]]) / 3.0;
+    const double cy = (m->py[q[0]] + m->py[q[1]] + m->py[q[2]]) / 3.0;
+    const uint64_t ix = static_cast<uint64_t>((cx - bx0) / (bx1 - bx0 + 1e-300) * (hn - 1));
+    const uint64_t iy = static_cast<uint64_t>((cy - by0) / (by1 - by0 + 1e-300) * (hn - 1));
+    order[t] = {hilbert_d(hn, ix, iy), t};
+  }
+  std::sort(order.begin(), order.end());
+  std::vector<Tri> tris;
+  tris.reserve(order.size());
+  for (const auto& o : order) {
+    const int32_t* q = prm->tri + 3 * static_cast<size_t>(o.second);
+    int32_t a = q[0], b = q[1], c = q[2];
+    const V2 B0 = sub(P(b), P(a)), C0 = sub(P(c), P(a));
+    if (B0.x * C0.y - B0.y * C0.x < 0) std::swap(b, c);
+    Tri t;
+    t.p[0] = a;
+    t.p[1] = b;
+    t.p[2] = c;
+    const V2 p0 = P(a), B = sub(P(b), p0), C = sub(P(c), p0);
+    const double d = 2.0 * (B.x * C.y - B.y * C.x);
+    const double b2 = B.x * B.x + B.y * B.y, c2 = C.x * C.x + C.y * C.y;
+    const double ux = (C.y * b2 - B.y * c2) / d;
+    const double uy = (B.x * c2 - C.x * b2) / d;
+    t.cc = V2{p0.x + ux, p0.y + uy};
+    t.r = std::sqrt(ux * ux + uy * uy);
+    tris.push_back(t);
+  }
+  // shared edges: the first triangle (in the order above) to list an edge
+  // owns it; the second one closes it
+  std::unordered_map<uint64_t, std::pair<int32_t, int32_t>> open;  // key -> (triangle, slot)
+  open.reserve(tris.size() * 2);
+  std::vector<EdgeRef> edges;
+  edges.reserve(tris.size() * 3 / 2);
+  for (int32_t t = 0; t < static_cast<int32_t>(tris.size()); ++t)
+    for (int k = 0; k < 3; ++k) {
+      const int32_t u = tris[t].p[k], w = tris[t].p[(k + 1) % 3];
+      const uint64_t key = (static_cast<uint64_t>(std::min(u, w)) << 32) |
+                           static_cast<uint32_t>(std::max(u, w));
+      auto it = open.find(key);
+      if (it == open.end()) {
+        open.emplace(key, std::make_pair(t, k));
+        continue;
+      }
+      const int32_t tlo = it->second.first, klo = it->second.second;
+      open.erase(it);
+      const int32_t a = tris[tlo].p[klo], b = tris[tlo].p[(klo + 1) % 3];
+      if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
+      edges.push_back(EdgeRef{tlo, t, a, b, klo});
+    }
+  finish_mesh(m, tris, edges, 0, ghost_wall, prm->x0, prm->x1, prm->y0, prm->y1);
   return m;
 }
 

@@ -20,11 +20,22 @@ class _LatticeParams(ctypes.Structure):
                 ("bc", ctypes.c_int32), ("flip_seed", ctypes.c_int32)]
 
 
+class _PointsParams(ctypes.Structure):
+    _fields_ = [("n_real", ctypes.c_int32), ("n_ghost", ctypes.c_int32),
+                ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("ghost_orig", ctypes.c_void_p), ("ghost_wall", ctypes.c_void_p),
+                ("n_tri", ctypes.c_int32), ("tri", ctypes.c_void_p),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double)]
+
+
 def _lib():
     lib = _ffi.load("librichmesh")
     if not getattr(lib, "_typed", False):
         lib.rm_lattice_build.restype = ctypes.c_void_p
         lib.rm_lattice_build.argtypes = [ctypes.POINTER(_LatticeParams)]
+        lib.rm_points_build.restype = ctypes.c_void_p
+        lib.rm_points_build.argtypes = [ctypes.POINTER(_PointsParams)]
         lib.rm_free.argtypes = [ctypes.c_void_p]
         lib.rm_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
         lib.rm_export.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 17
@@ -98,6 +109,61 @@ def lattice(nx, ny, box=(0.0, 1.0, 0.0, 1.0), jitter=1e-3, seed=0, bc=RIGID, fli
     h = lib.rm_lattice_build(ctypes.byref(p))
     if not h:
         raise ValueError("rm_lattice_build rejected the parameters")
+    return _export(h, bc, lattice_ij=True)
+
+
+def points(x, y, box, margin=None):
+    """Voronoi mesh of arbitrary points in the rigid-wall box (x0, x1, y0,
+    y1), e.g. after the points moved.  Ghosts are the reflections across each
+    wall of the points within `margin` of it (default: 3 mean spacings), as
+    the lattice builder reflects its outer rows; the Delaunay triangulation of
+    points and ghosts comes from Qhull (scipy.spatial.Delaunay) and
+    rm_points_build forms faces and geometry by the lattice builder's rules.
+    Cells keep the order of the points."""
+    from scipy.spatial import Delaunay
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    N = len(x)
+    x0, x1, y0, y1 = box
+    if margin is None:
+        margin = 3.0 * np.sqrt((x1 - x0) * (y1 - y0) / N)
+    gx, gy, go, gw = [], [], [], []
+    for wall, near, fx, fy in (
+            (0, x1 - x < margin, lambda: x + 2 * (x1 - x), lambda: y + 0.0),
+            (1, y1 - y < margin, lambda: x + 0.0, lambda: y + 2 * (y1 - y)),
+            (2, x - x0 < margin, lambda: x + 2 * (x0 - x), lambda: y + 0.0),
+            (3, y - y0 < margin, lambda: x + 0.0, lambda: y + 2 * (y0 - y))):
+        idx = np.nonzero(near)[0]
+        gx.append(fx()[idx])
+        gy.append(fy()[idx])
+        go.append(idx.astype(np.int32))
+        gw.append(np.full(len(idx), wall, np.int32))
+    X = np.concatenate([x] + gx)
+    Y = np.concatenate([y] + gy)
+    gorig = np.ascontiguousarray(np.concatenate(go), dtype=np.int32)
+    gwall = np.ascontiguousarray(np.concatenate(gw), dtype=np.int32)
+    tri = Delaunay(np.column_stack([X, Y])).simplices.astype(np.int64)
+    # keep the triangles with a real vertex and the ghosts they use
+    tri = tri[(tri < N).any(axis=1)]
+    used = np.unique(tri[tri >= N]) - N
+    new_id = np.full(len(gorig), -1, np.int64)
+    new_id[used] = N + np.arange(len(used))
+    tri = np.where(tri >= N, new_id[np.maximum(tri - N, 0)], tri)
+    X = np.ascontiguousarray(np.concatenate([x, X[N:][used]]))
+    Y = np.ascontiguousarray(np.concatenate([y, Y[N:][used]]))
+    gorig = np.ascontiguousarray(gorig[used])
+    gwall = np.ascontiguousarray(gwall[used])
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    p = _PointsParams(N, len(gorig), _ffi.ptr(X), _ffi.ptr(Y), _ffi.ptr(gorig), _ffi.ptr(gwall),
+                      len(tri), _ffi.ptr(tri), x0, x1, y0, y1)
+    h = _lib().rm_points_build(ctypes.byref(p))
+    if not h:
+        raise ValueError("rm_points_build rejected the input")
+    return _export(h, RIGID, lattice_ij=False)
+
+
+def _export(h, bc, lattice_ij):
+    lib = _lib()
     handle = _Handle(h)
     n = [ctypes.c_int32() for _ in range(4)]
     lib.rm_sizes(h, *[ctypes.byref(x) for x in n])
@@ -107,8 +173,9 @@ def lattice(nx, ny, box=(0.0, 1.0, 0.0, 1.0), jitter=1e-3, seed=0, bc=RIGID, fli
     arrs = dict(face_first=i32(F), face_second=i32(F), v1x=f64(F), v1y=f64(F), v2x=f64(F),
                 v2y=f64(F), pt_x=f64(N + G), pt_y=f64(N + G), cm_x=f64(N + G),
                 cm_y=f64(N + G), volume=f64(N), tess_volume=f64(N), csr_off=i32(N + 1),
-                csr_face=i32(nnz), ghost_orig=i32(G), lat_i=i32(N), lat_j=i32(N))
-    lib.rm_export(h, *[_ffi.ptr(arrs[k]) for k in (
+                csr_face=i32(nnz), ghost_orig=i32(G),
+                lat_i=i32(N) if lattice_ij else None, lat_j=i32(N) if lattice_ij else None)
+    lib.rm_export(h, *[_ffi.ptr(arrs[k]) if arrs[k] is not None else None for k in (
         "face_first", "face_second", "v1x", "v1y", "v2x", "v2y", "pt_x", "pt_y", "cm_x", "cm_y",
         "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig", "lat_i", "lat_j")])
     return Mesh(n_cells=N, n_faces=F, n_ghosts=G, bc=bc, _handle=handle, **arrs)

@@ -176,10 +176,12 @@ int rg_adjoint_backward(rg_ctx* ctx, const double* gcells, const double* gext, d
  * [nsteps][n_inst]) and runs each step as one hipGraph launch of persistent
  * kernels; integrator 2, order 2, RG_FP64, zero face velocities. */
 int rg_set_instances(rg_ctx* ctx, int32_t n_inst, const int32_t* cell_off);
-/* Launch shape of the persistent kernels: block (face/cell tile) 64, 128
- * or 256 threads and blocks_per_cu resident blocks per compute unit. */
+/* Launch shape of the step kernels: block (face/cell tile) 64, 128 or 256
+ * threads and blocks_per_cu resident blocks per compute unit walking the
+ * tiles (persistent), or blocks_per_cu = 0 for one thread per cell/face. */
 int rg_ensemble_config(rg_ctx* ctx, int32_t block, int32_t blocks_per_cu);
-/* Times trial_steps steps for every launch shape, keeps the fastest and
+/* Times trial_steps steps for every launch shape (blocks_per_cu 1..16 and
+ * 0), keeps the fastest and
  * restores the state; table[3*r..] = (block, blocks_per_cu, ms per step)
  * for up to table_rows rows; *rows_out = rows measured. */
 int rg_ensemble_autotune(rg_ctx* ctx, int32_t trial_steps, double* table, int32_t table_rows,

@@ -56,6 +56,25 @@ typedef struct rm_mesh rm_mesh;
 rm_mesh* rm_lattice_build(const rm_lattice_params* p);
 void rm_free(rm_mesh* m);
 
+/* A mesh from arbitrary points in a rigid-wall box [x0,x1] x [y0,y1]: the
+ * caller supplies the real points [0, n_real), ghost points reflected across
+ * a wall (ghost_orig: the real point reflected; ghost_wall: 0 right, 1 up,
+ * 2 left, 3 down) and the Delaunay triangles of all of them (any
+ * orientation).  Triangles are numbered in Hilbert order of their centroids;
+ * faces, their order and the cell geometry then follow the same rules as
+ * rm_lattice_build.  Returns NULL on bad input. */
+typedef struct rm_points_params {
+  int32_t n_real, n_ghost;
+  const double *x, *y;            /* [n_real + n_ghost]                  */
+  const int32_t *ghost_orig;      /* [n_ghost]                           */
+  const int32_t *ghost_wall;      /* [n_ghost]                           */
+  int32_t n_tri;
+  const int32_t *tri;             /* [3 n_tri] point ids                 */
+  double x0, x1, y0, y1;
+} rm_points_params;
+
+rm_mesh* rm_points_build(const rm_points_params* p);
+
 /* Sizes: cells N, faces F, ghosts G, CSR entries nnz. */
 void rm_sizes(const rm_mesh* m, int32_t* n_cells, int32_t* n_faces,
               int32_t* n_ghosts, int32_t* nnz);

@@ -63,7 +63,7 @@ def test_ensemble_equals_instances_alone():
     np.testing.assert_array_equal(dts[:, 0], d0)
 
 
-@pytest.mark.parametrize("shape", [(64, 1), (128, 3), (256, 16)])
+@pytest.mark.parametrize("shape", [(64, 1), (128, 3), (256, 16), (128, 0)])
 def test_launch_shape_does_not_change_results(shape):
     ms, cells, prm, u = _ensemble(K=4, seed=5)
     base, bdts, _ = _batched(u, cells, 6)
