@@ -101,6 +101,48 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def moving_leg(args):
+    """Sedov on a moving mesh: Lagrangian points with RoundCells, the Voronoi
+    mesh rebuilt on the host every step, the plugins on the GPU."""
+    import numpy as np
+    import cases
+    from rich_amd.moving import GpuBackend, MovingMesh
+
+    n = args.moving_n
+    log(f"moving mesh: Sedov {n}x{n}, {args.moving_steps} steps")
+    m = cases.sedov_mesh(n)
+    c = cases.sedov_cells(m)
+    N = m.n_cells
+    t0 = time.perf_counter()
+    run = MovingMesh(m.pt_x[:N], m.pt_y[:N], c, (-1.0, 1.0, -1.0, 1.0), GpuBackend())
+    setup = time.perf_counter() - t0
+    x0 = run.ext.copy()
+    run.timing = {k: 0.0 for k in run.timing}
+    t0 = time.perf_counter()
+    for _ in range(args.moving_steps):
+        run.step()
+    el = time.perf_counter() - t0
+    steps = args.moving_steps
+    T = run.timing
+    out = {"workload": f"Sedov {n}x{n} jittered lattice in [-1,1]^2, rigid walls, Lagrangian + "
+                       "RoundCells points, Voronoi rebuilt on the host every step, "
+                       "LinearGaussImproved + HLLC, Heun, FP64",
+           "cells": N, "steps": steps, "sim_time": run.time,
+           "s_per_step": el / steps, "value": N * steps / el / 1e6, "unit": "Mcell-updates/s",
+           "host_mesh_s_per_step": T["mesh_s"] / steps,
+           "host_motion_s_per_step": T["motion_s"] / steps,
+           "gpu_plugins_s_per_step": T["plugins_s"] / steps,
+           "gpu_plugins_value": N * steps / T["plugins_s"] / 1e6,
+           "mass_drift": float(run.ext[0].sum() / x0[0].sum() - 1.0),
+           "energy_drift": float(run.ext[1].sum() / x0[1].sum() - 1.0),
+           "setup_s": setup,
+           "note": "plugin time includes the host<->device copies of the plugin-level C ABI "
+                   "(cells, extensives and fluxes each way, twice per step) and one context "
+                   "and mesh upload per step"}
+    run.b.close()
+    return out
+
+
 def ensemble_leg(args):
     """256 independent Sedov runs of uneven size advanced together (one
     hipGraph launch of persistent kernels per step, per-instance dt, launch
@@ -276,6 +318,11 @@ def main():
     ap.add_argument("--adjoint-cpu-ref", type=int, default=8, metavar="K",
                     help="also compare K directional derivatives with the forward-mode CPU "
                          "oracle (dual numbers, OpenMP, 8 per pass); 0 = skip")
+    ap.add_argument("--no-moving", action="store_true", help="skip the moving-mesh leg")
+    ap.add_argument("--moving-n", type=int, default=512)
+    ap.add_argument("--moving-steps", type=int, default=3,
+                    help="steps of the moving-mesh Sedov run (the host Voronoi rebuild "
+                         "dominates; 200 is the full run)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-k", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -409,6 +456,8 @@ def main():
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}
     if mixed:
         line["mixed"] = mixed
+    if not args.no_moving:
+        line["moving_mesh"] = moving_leg(args)
     if not args.no_ensemble:
         line["ensemble"] = ensemble_leg(args)
     if not args.no_adjoint:
