@@ -119,8 +119,10 @@ def moving_leg(args):
     x0 = run.ext.copy()
     run.timing = {k: 0.0 for k in run.timing}
     t0 = time.perf_counter()
-    for _ in range(args.moving_steps):
+    for k in range(args.moving_steps):
         run.step()
+        if (k + 1) % 20 == 0:
+            log(f"moving mesh: step {k + 1}, {time.perf_counter() - t0:.1f} s")
     el = time.perf_counter() - t0
     steps = args.moving_steps
     T = run.timing
