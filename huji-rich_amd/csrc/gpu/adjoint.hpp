@@ -145,10 +145,11 @@ __global__ void __launch_bounds__(256) k_adj_flux(DMesh m, const D4* __restrict_
     const double dt = *dtp;
     const FaceFrame fr = m.fframe[f];
     const int a = m.ff[f], b = m.fs[f];
-    const Cell L = side_state(m, q, rec, a, fr.cx, fr.cy);
-    const Cell R = side_state(m, q, rec, b, fr.cx, fr.cy);
+    const Cell L = side_state(m, q, rec, a, fr.oxa, fr.oya);
+    const Cell R = side_state(m, q, rec, b, fr.oxb, fr.oyb);
+    const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
     Cons c;
-    (void)hllc_face(L, R, gamma, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap, 0.0, c);
+    (void)hllc_face(L, R, gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, 0.0, c);
     const double area = m.area[f];
     const D4 z{0, 0, 0, 0};
     const D4 ga = a < m.N ? gs[a] : z, gb = b < m.N ? gs[b] : z;
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(256) k_adj_flux(DMesh m, const D4* __restrict_
     const Cons fb{s * dg.a, s * dg.c, s * dg.d, s * dg.b};
     dtb = area * (c.m * dg.a + c.E * dg.b + c.mx * dg.c + c.my * dg.d);
     Cell Lb{0, 0, 0, 0}, Rb{0, 0, 0, 0};
-    hllc_face_vjp(L, R, gamma, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap, fb, Lb, Rb);
+    hllc_face_vjp(L, R, gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, fb, Lb, Rb);
     lb_out[f] = cell_d4(Lb);
     rb_out[f] = cell_d4(Rb);
   }

@@ -51,18 +51,19 @@ struct alignas(64) CellGeo {
 struct alignas(32) FaceRec {
   double hx, hy, cx, cy;
 };
+// the face centroid minus the CM of each side (first a, second b), the
+// normal n and the face direction p; |n| and |p| are recomputed with vabs
 struct alignas(64) FaceFrame {
-  double cx, cy, nx, ny, an, px, py, ap;
+  double oxa, oya, oxb, oyb, nx, ny, px, py;
 };
 struct alignas(8) Inc {
   int nbr;
   unsigned info;
 };
-struct alignas(128) Rec {
+struct alignas(32) Rec {  // a cell and its limited slopes
   double d, p, vx, vy;
   double xd, xp, xvx, xvy;
   double yd, yp, yvx, yvy;
-  double cmx, cmy, pad0, pad1;
 };
 
 // Raw mesh arrays (SoA, as uploaded) and the static per-mesh tables.
@@ -217,7 +218,10 @@ __global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceF
   const double s = qx * ppx + qy * ppy;
   const double pp = ppx * ppx + ppy * ppy;
   const double nx = qx - ppx * s / pp, ny = qy - ppy * s / pp;
-  fframe[f] = FaceFrame{cx, cy, nx, ny, vabs(nx, ny), ppx, ppy, vabs(ppx, ppy)};
+  // interp's centroid minus CM (LinearGaussImproved.cpp:159-172), for the
+  // ghost sides at the ghost's CM (:597-634)
+  const D2 ca = m.cm[a], cb = m.cm[b];
+  fframe[f] = FaceFrame{cx - ca.x, cy - ca.y, cx - cb.x, cy - cb.y, nx, ny, ppx, ppy};
   // SlabSymmetry::calcArea (physical_geometry.cpp:8-11)
   area[f] = vabs(v1x - v2x, v1y - v2y);
 }
@@ -366,7 +370,7 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
   Cell amax = cell, amin = cell, bmax = cell, bmin = cell;
   if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
     report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
-    out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0, g.cmx, g.cmy, 0, 0};
+    out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
   if (kind == KIND_BOUNDARY) {
@@ -486,8 +490,8 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
     sx.vy *= psi3;
     sy.vy *= psi3;
   }
-  out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, sx.d,  sx.p,  sx.vx, sx.vy,
-               sy.d,   sy.p,   sy.vx,   sy.vy,   g.cmx, g.cmy, 0,     0};
+  out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, sx.d, sx.p, sx.vx, sx.vy,
+               sy.d,   sy.p,   sy.vx,   sy.vy};
 }
 
 __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
@@ -505,23 +509,17 @@ __global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict_
 // Face state on one side (LinearGaussImproved.cpp:571-590 for real cells,
 // :597-634 for ghosts: the ghost value plus the ghost gradient -- zero for
 // rigid walls, the original's slope for periodic boxes -- at the ghost's CM).
+// (ox, oy) = face centroid minus that side's CM, as interp forms it.
 __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Rec* rec, int id,
-                                           double fx, double fy) {
-  if (id < m.N) {
-    const Rec r = rec[id];
+                                           double ox, double oy) {
+  if (id < m.N || m.bc == BC_PERIODIC) {
+    const Rec r = rec[id < m.N ? id : m.gorig[id - m.N]];
     const Cell v{r.d, r.p, r.vx, r.vy};
     const Cell sx{r.xd, r.xp, r.xvx, r.xvy}, sy{r.yd, r.yp, r.yvx, r.yvy};
-    return interp(v, sx, sy, fx, fy, r.cmx, r.cmy);
-  }
-  const D2 cg = m.cm[id];
-  if (m.bc == BC_PERIODIC) {
-    const Rec r = rec[m.gorig[id - m.N]];
-    const Cell v{r.d, r.p, r.vx, r.vy};
-    const Cell sx{r.xd, r.xp, r.xvx, r.xvy}, sy{r.yd, r.yp, r.yvx, r.yvy};
-    return interp(v, sx, sy, fx, fy, cg.x, cg.y);
+    return interp_off(v, sx, sy, ox, oy);
   }
   const Cell z{0, 0, 0, 0};
-  return interp(fetch(m, q, id), z, z, fx, fy, cg.x, cg.y);
+  return interp_off(fetch(m, q, id), z, z, ox, oy);
 }
 
 __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict__ q,
@@ -529,13 +527,14 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
                                            const DParams& prm, const DErr& err, int f) {
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
-  const Cell lf = side_state(m, q, rec, m.ff[f], fr.cx, fr.cy);
-  const Cell rt = side_state(m, q, rec, m.fs[f], fr.cx, fr.cy);
+  const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
+  const Cell rt = side_state(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
+  const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
   // modular_flux_calculator.cpp:56-67
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
-  const double speed = (fr.nx * wx + fr.ny * wy) / fr.an;
+  const double speed = (fr.nx * wx + fr.ny * wy) / an;
   Cons c;
-  const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, fr.an, fr.px, fr.py, fr.ap, speed, c);
+  const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
   D4 o{0, 0, 0, 0};
   if (rc)
     report(err, f, rc);
@@ -564,8 +563,8 @@ __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restri
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
-  const Cell lf = side_state(m, q, rec, m.ff[f], fr.cx, fr.cy);
-  const Cell rt = side_state(m, q, rec, m.fs[f], fr.cx, fr.cy);
+  const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
+  const Cell rt = side_state(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
   left[f] = D4{lf.d, lf.p, lf.vx, lf.vy};
   right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
 }

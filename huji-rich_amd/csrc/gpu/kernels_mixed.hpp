@@ -122,7 +122,8 @@ __global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
   const FaceRec r = m.frec[f];
   const FaceFrame fr = m.fframe[f];
   const D2 ca = m.cm[m.ff[f]], cb = m.cm[m.fs[f]];
-  face[f] = FaceM{static_cast<float>(fr.nx / fr.an), static_cast<float>(fr.ny / fr.an),
+  const double an = vabs(fr.nx, fr.ny);
+  face[f] = FaceM{static_cast<float>(fr.nx / an), static_cast<float>(fr.ny / an),
                   static_cast<float>(m.area[f]), 0.f,
                   static_cast<float>(r.cx - ca.x), static_cast<float>(r.cy - ca.y),
                   static_cast<float>(r.cx - cb.x), static_cast<float>(r.cy - cb.y)};
