@@ -522,9 +522,13 @@ __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Re
   return interp_off(fetch(m, q, id), z, z, ox, oy);
 }
 
+// With dt > 0 the face writes its extensive update dt*area*F (the products
+// SimpleExtensiveUpdater forms, simple_extensive_updater.cpp:23-34) instead
+// of F.
 __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict__ q,
                                            const Rec* __restrict__ rec, D4* __restrict__ flux,
-                                           const DParams& prm, const DErr& err, int f) {
+                                           const DParams& prm, const DErr& err, int f,
+                                           double dt = 0.0) {
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
   const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
@@ -536,18 +540,25 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
   Cons c;
   const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
   D4 o{0, 0, 0, 0};
-  if (rc)
+  if (rc) {
     report(err, f, rc);
-  else
+  } else if (dt > 0) {
+    const double sc = dt * m.area[f];
+    o = D4{c.m * sc, c.E * sc, c.mx * sc, c.my * sc};
+  } else {
     o = D4{c.m, c.E, c.mx, c.my};
+  }
   flux[f] = o;
 }
 
-// SpatialReconstruction output res[f].first / .second
+// dt_slot non-null: write dt*area*F with dt = cfl * *dt_slot (the step loop)
 __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
                                                const Rec* __restrict__ rec, D4* __restrict__ flux,
-                                               DParams prm, DErr err) {
-  flux2_face(m, q, rec, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x);
+                                               DParams prm, DErr err,
+                                               const unsigned long long* __restrict__ dt_slot) {
+  const double dt =
+      dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*dt_slot)) : 0.0;
+  flux2_face(m, q, rec, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x, dt);
 }
 // Persistent form: a fixed grid walks the faces in tiles of blockDim.x.
 __global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__ q,
@@ -557,6 +568,7 @@ __global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__
     flux2_face(m, q, rec, flux, prm, err, f);
 }
 
+// SpatialReconstruction output res[f].first / .second
 __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restrict__ q,
                                                      const Rec* __restrict__ rec, D4* left,
                                                      D4* right) {
@@ -656,6 +668,7 @@ enum : int {
   UPD_CELLS = 8,      // SimpleCellUpdater
   UPD_CFL = 16,       // next step's CFL term
   UPD_FACES = 32,     // apply the face fluxes
+  UPD_DELTA = 64,     // the face array holds dt*area*F already
 };
 
 struct UpdArgs {
@@ -698,11 +711,12 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
         unsigned inf[CH];
         D4 fl[CH];
         double ar[CH];
+        const bool delta = (a.flags & UPD_DELTA) != 0;
 #pragma unroll
         for (int u = 0; u < CH; ++u)
           if (s0 + u < deg) {
             inf[u] = m.einfo[eb + 64 * (s0 + u)];
-            ar[u] = m.earea[eb + 64 * (s0 + u)];
+            ar[u] = delta ? 0.0 : m.earea[eb + 64 * (s0 + u)];
           }
 #pragma unroll
         for (int u = 0; u < CH; ++u)
@@ -710,8 +724,19 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
           if (s0 + u >= deg) break;
-          const double s = dt * ar[u];
-          const double dm = fl[u].a * s, dE = fl[u].b * s, dx = fl[u].c * s, dy = fl[u].d * s;
+          double dm, dE, dx, dy;
+          if (delta) {
+            dm = fl[u].a;
+            dE = fl[u].b;
+            dx = fl[u].c;
+            dy = fl[u].d;
+          } else {
+            const double s = dt * ar[u];
+            dm = fl[u].a * s;
+            dE = fl[u].b * s;
+            dx = fl[u].c * s;
+            dy = fl[u].d * s;
+          }
           if (inf[u] & INFO_SECOND) {
             em += dm;
             eE += dE;

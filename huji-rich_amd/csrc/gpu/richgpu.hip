@@ -297,14 +297,16 @@ hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out 
 }
 
 // FluxCalculator on `cells` into c->flux (slopes into `rec`, default c->rec).
-hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* rec = nullptr) {
+// With a dt slot, second order writes dt*area*F (for UPD_DELTA).
+hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* rec = nullptr,
+                    const unsigned long long* dt_slot = nullptr) {
   if (c->prm.order == 2) {
     Rec* r = rec ? rec : c->rec;
     const hipError_t e = run_slopes(c, cells, step, phase_base, r);
     if (e != hipSuccess) return e;
     return launch(c, KID_FLUX2, [&] {
       k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, r, c->flux, dparams(c),
-                                                           derr(c, step, phase_base + 1));
+                                                           derr(c, step, phase_base + 1), dt_slot);
     });
   }
   return launch(c, KID_FLUX1, [&] {
@@ -378,7 +380,10 @@ HeunBufs ctx_bufs(rg_ctx* c) {
 // CFL terms into slot (s+1)&1.  dt_hist[hist] receives the step's dt.
 hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs& h) {
   const int cur = s & 1, nxt = (s + 1) & 1;
-  hipError_t e = run_flux(c, h.P, s, 1, h.r1);
+  // second order writes dt*area*F straight from the flux kernel
+  const int delta = c->prm.order == 2 ? UPD_DELTA : 0;
+  const unsigned long long* slot = delta ? c->words + cur : nullptr;
+  hipError_t e = run_flux(c, h.P, s, 1, h.r1, slot);
   if (e != hipSuccess) return e;
   UpdArgs a{};
   a.in = h.X;
@@ -388,12 +393,12 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   a.dt_slot = c->words + cur;
   a.dt_hist = dt_hist;
   a.step = hist;
-  a.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_CELLS;
+  a.flags = UPD_FACES | delta | UPD_ZF_BEFORE | UPD_CELLS;
   e = launch(c, KID_UPDATE, [&] {
     k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c), derr(c, s, 3));
   });
   if (e != hipSuccess) return e;
-  e = run_flux(c, h.Pm, s, 4, h.r2);
+  e = run_flux(c, h.Pm, s, 4, h.r2, slot);
   if (e != hipSuccess) return e;
   UpdArgs b{};
   b.in = h.Xm;
@@ -404,7 +409,7 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   b.dt_slot = c->words + cur;
   b.partials = c->partials;
   b.step = hist;
-  b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
+  b.flags = UPD_FACES | delta | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   e = launch(c, KID_UPDATE, [&] {
     k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, b, dparams(c), derr(c, s, 6));
   });
