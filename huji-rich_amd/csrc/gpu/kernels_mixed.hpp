@@ -42,7 +42,8 @@ struct MTables {
   const CellGeoM* cgeo;
   const int* wofs;
   const int* enbr;       // neighbour id (ghost ids as in the mesh)
-  const F4* egeo;        // (hx, hy) = r_ij/2 seen from the cell, (ox, oy) = centroid - cell CM
+  const float2* eh;      // r_ij/2 seen from the cell (gradient pass)
+  const float2* eo;      // centroid - cell CM (limiter pass)
   const unsigned* einfo; // flux to read: face | INFO_SECOND, periodic copies shared
 };
 
@@ -131,7 +132,8 @@ __global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
 
 // ELL geometry of every incidence: one thread per cell over its faces.
 __global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, const int* __restrict__ wofs,
-                                                       F4* __restrict__ egeo) {
+                                                       float2* __restrict__ eh,
+                                                       float2* __restrict__ eo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const D2 ci = m.cm[i];
@@ -140,9 +142,10 @@ __global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, const int* __res
     const unsigned info = m.inc[k].info;
     const FaceRec r = m.frec[info & INFO_FACE];
     const bool second = (info & INFO_SECOND) != 0;
-    egeo[base + 64 * (k - m.off[i])] =
-        F4{static_cast<float>(second ? -r.hx : r.hx), static_cast<float>(second ? -r.hy : r.hy),
-           static_cast<float>(r.cx - ci.x), static_cast<float>(r.cy - ci.y)};
+    const int at = base + 64 * (k - m.off[i]);
+    eh[at] = make_float2(static_cast<float>(second ? -r.hx : r.hx),
+                         static_cast<float>(second ? -r.hy : r.hy));
+    eo[at] = make_float2(static_cast<float>(r.cx - ci.x), static_cast<float>(r.cy - ci.y));
   }
 }
 
@@ -197,15 +200,14 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
   } else {
     CellF vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
     for (int s0 = 0; s0 < deg; s0 += CH) {
-      float4 hh[CH];
+      float2 hh[CH];
       CellF nc[CH];
       int nb[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
         if (s0 + u < deg) {
           nb[u] = t.enbr[eb + 64 * (s0 + u)];
-          const F4 gg = t.egeo[eb + 64 * (s0 + u)];
-          hh[u] = make_float4(gg.a, gg.b, gg.c, gg.d);
+          hh[u] = t.eh[eb + 64 * (s0 + u)];
         }
 #pragma unroll
       for (int u = 0; u < CH; ++u)
@@ -239,10 +241,7 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
       float2 oo[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) {
-          const F4 gg = t.egeo[eb + 64 * (s0 + u)];
-          oo[u] = make_float2(gg.c, gg.d);
-        }
+        if (s0 + u < deg) oo[u] = t.eo[eb + 64 * (s0 + u)];
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         if (s0 + u >= deg) break;

@@ -900,13 +900,15 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) einfoM[ell_at(i, k)] = minfo[k];
     FaceM* faceM;
     CellGeoM* cgeoM;
-    F4* degeo;
+    float2 *deh, *deo;
     unsigned* deinfo;
     HIPCHK(c, dalloc(c, &faceM, F));
     HIPCHK(c, dalloc(c, &cgeoM, N));
     HIPCHK(c, upload(c, &deinfo, einfoM.data(), ne));
-    HIPCHK(c, dalloc(c, &degeo, ne));
-    HIPCHK(c, hipMemsetAsync(degeo, 0, sizeof(F4) * ne, c->stream));
+    HIPCHK(c, dalloc(c, &deh, ne));
+    HIPCHK(c, dalloc(c, &deo, ne));
+    HIPCHK(c, hipMemsetAsync(deh, 0, sizeof(float2) * ne, c->stream));
+    HIPCHK(c, hipMemsetAsync(deo, 0, sizeof(float2) * ne, c->stream));
     HIPCHK(c, dalloc(c, &c->primM, N));
     HIPCHK(c, dalloc(c, &c->mid_primM, N));
     HIPCHK(c, dalloc(c, &c->fluxM, F));
@@ -918,10 +920,10 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       k_geo_mixed_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeoM);
     }));
     HIPCHK(c, launch(c, KID_GEO, [&] {
-      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, wofs_d, degeo);
+      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, wofs_d, deh, deo);
     }));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above are uploaded
-    c->mt = MTables{faceM, cgeoM, wofs_d, enbr_d, degeo, deinfo};
+    c->mt = MTables{faceM, cgeoM, wofs_d, enbr_d, deh, deo, deinfo};
   }
   c->dm = d;
   c->N = N;
