@@ -15,6 +15,7 @@
 
 #include "adjoint.hpp"
 #include "ensemble.hpp"
+#include "moving.hpp"
 #include "kernels_mixed.hpp"
 
 using namespace rg;
@@ -40,6 +41,7 @@ enum KernelId : int {
   KID_ADJ_SLOPES,
   KID_ADJ_CELL,
   KID_ENS,
+  KID_MOVING,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -47,7 +49,8 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_init_ext",    "k_fill_u64", "k_geo",
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
-                                             "k_adj_slopes",  "k_adj_cell",     "k_ens"};
+                                             "k_adj_slopes",  "k_adj_cell",     "k_ens",
+                                             "k_moving"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -120,6 +123,11 @@ struct rg_ctx {
   unsigned long long *ens_dtA = nullptr, *ens_dtB = nullptr;
   double *ens_inv = nullptr, *ens_hist = nullptr;
   hipGraphExec_t ens_graph = nullptr;
+
+  // moving mesh: point velocities kept across rg_replace_mesh
+  D2* pv = nullptr;
+  int pv_cap = 0;
+  bool pv_valid = false;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -583,6 +591,7 @@ void rg_destroy(rg_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
+  if (c->pv) (void)hipFree(c->pv);
   if (c->dt_hist) (void)hipFree(c->dt_hist);
   if (c->words) (void)hipFree(c->words);
   if (c->host_words) (void)hipHostFree(c->host_words);
@@ -1441,6 +1450,143 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
   c->prof = prof;
   HIPCHK(c, ens_capture(c, best_block, best_bpc));
   if (rows_out) *rows_out = rows;
+  return collect_error(c);
+}
+
+int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {
+  if (!c || !m || !c->has_mesh || !c->has_cells || !c->has_ext || m->n_cells != c->N)
+    return RG_E_ARG;
+  const size_t bytes = sizeof(D4) * static_cast<size_t>(c->N);
+  D4* keep = nullptr;  // cells, extensives, mid extensives
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&keep), 3 * bytes));
+  hipError_t e = hipMemcpyAsync(keep, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(keep + c->N, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(keep + 2 * static_cast<size_t>(c->N), c->mid_ext, bytes,
+                       hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(keep);
+    return fail_hip(c, e);
+  }
+  const bool pv_valid = c->pv_valid;
+  const int rc = rg_set_mesh(c, m);
+  if (rc == RG_OK) {
+    e = hipMemcpyAsync(c->prim, keep, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->ext, keep + c->N, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(c->mid_ext, keep + 2 * static_cast<size_t>(c->N), bytes,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  (void)hipFree(keep);
+  if (rc != RG_OK) return rc;
+  if (e != hipSuccess) return fail_hip(c, e);
+  c->has_cells = c->has_ext = true;
+  c->pv_valid = pv_valid;
+  return RG_OK;
+}
+
+namespace {
+
+// StationaryBox face velocities of the current mesh from the device point
+// velocities into the context's face-velocity buffer.
+hipError_t moving_faces(rg_ctx* c) {
+  if (!c->wbuf) {
+    const hipError_t e = dalloc(c, &c->wbuf, 2 * static_cast<size_t>(c->F));
+    if (e != hipSuccess) return e;
+  }
+  c->dm.wx = c->wbuf;
+  c->dm.wy = c->wbuf + c->F;
+  return launch(c, KID_MOVING, [&] {
+    k_face_velocity<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->pv, c->wbuf,
+                                                                c->wbuf + c->F);
+  });
+}
+
+}  // namespace
+
+int rg_moving_begin(rg_ctx* c, double chi, double eta, const double* box, double* dt_out,
+                    double* pvx, double* pvy) {
+  if (!ready(c, true, true) || c->prm.order != 2 || c->prm.bc != RG_BC_RIGID ||
+      c->prm.precision != RG_FP64 || c->ens_K > 0 || !box || !dt_out)
+    return RG_E_ARG;
+  const int N = c->N;
+  if (c->pv_cap < N) {
+    if (c->pv) (void)hipFree(c->pv);
+    c->pv = nullptr;
+    c->pv_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->pv), sizeof(D2) * N));
+    c->pv_cap = N;
+  }
+  c->ext_init = false;
+  // hdsim2d.cpp:325-344: point velocities, face velocities, dt; ApplyFix;
+  // face velocities and dt again
+  HIPCHK(c, launch(c, KID_MOVING, [&] {
+    k_point_velocity<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->prim, c->prm.gamma, chi,
+                                                             eta, c->pv);
+  }));
+  HIPCHK(c, moving_faces(c));
+  HIPCHK(c, run_cfl(c, c->prim, 0));
+  HIPCHK(c, launch(c, KID_MOVING, [&] {
+    k_point_fix<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->pv, c->words, c->prm.cfl,
+                                                        box[0], box[1], box[2], box[3]);
+  }));
+  HIPCHK(c, moving_faces(c));
+  HIPCHK(c, run_cfl(c, c->prim, 0));
+  // stage 1 on the old mesh (hdsim2d.cpp:346-371): mid = ext + ZeroForce + fluxes
+  HIPCHK(c, run_flux(c, c->prim, 0, 1, c->rec, c->words));
+  UpdArgs a{};
+  a.in = c->ext;
+  a.out = c->mid_ext;
+  a.flux = c->flux;
+  a.dt_slot = c->words;
+  a.flags = UPD_FACES | UPD_DELTA | UPD_ZF_BEFORE;
+  HIPCHK(c, run_update(c, a, 0, 3));
+  unsigned long long w = 0;
+  HIPCHK(c, hipMemcpyAsync(&w, c->words, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  std::vector<D2> h;
+  if (pvx || pvy) {
+    h.resize(static_cast<size_t>(N));
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->pv, sizeof(D2) * N, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double inv;
+  std::memcpy(&inv, &w, sizeof(inv));
+  *dt_out = c->prm.cfl * inv;
+  for (int i = 0; i < N && !h.empty(); ++i) {
+    if (pvx) pvx[i] = h[i].x;
+    if (pvy) pvy[i] = h[i].y;
+  }
+  const int rc = collect_error(c);
+  c->pv_valid = rc == RG_OK;
+  return rc;
+}
+
+int rg_moving_end(rg_ctx* c) {
+  if (!ready(c, true, true) || !c->pv_valid || c->pv_cap < c->N) return RG_E_ARG;
+  c->pv_valid = false;
+  // mid cells with the new volumes (hdsim2d.cpp:401-404)
+  UpdArgs u{};
+  u.in = c->mid_ext;
+  u.out = c->mid_ext;
+  u.cells = c->mid_prim;
+  u.flags = UPD_CELLS;
+  HIPCHK(c, run_update(c, u, 0, 3));
+  // stage 2 on the new mesh, the average and the cell update (:410-441)
+  HIPCHK(c, moving_faces(c));
+  HIPCHK(c, run_flux(c, c->mid_prim, 0, 4, c->rec, c->words));
+  UpdArgs b{};
+  b.in = c->mid_ext;
+  b.out = c->ext;
+  b.avg = c->ext;
+  b.flux = c->flux;
+  b.cells = c->prim;
+  b.dt_slot = c->words;
+  b.flags = UPD_FACES | UPD_DELTA | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS;
+  HIPCHK(c, run_update(c, b, 0, 6));
   return collect_error(c);
 }
 

@@ -61,6 +61,10 @@ def lib():
     L.rg_set_stream.argtypes = [vp, vp]
     L.rg_set_mesh.argtypes = [vp, P(MeshS)]
     L.rg_set_face_velocity.argtypes = [vp, vp, vp]
+    L.rg_replace_mesh.argtypes = [vp, P(MeshS)]
+    L.rg_moving_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp,
+                                  P(ctypes.c_double), vp, vp]
+    L.rg_moving_end.argtypes = [vp]
     for n in ("rg_set_cells", "rg_get_cells", "rg_set_extensives", "rg_get_extensives",
               "rg_set_fluxes", "rg_flux"):
         getattr(L, n).argtypes = [vp, vp, vp, vp, vp]
@@ -110,6 +114,9 @@ class RichGPU:
         h = ctypes.c_void_p()
         self._check(self._lib.rg_create(ctypes.byref(prm), ctypes.byref(h)), handle=None)
         self._h = h
+        self._upload(self._lib.rg_set_mesh, mesh)
+
+    def _upload(self, fn, mesh):
         keep = []
         ptrs = []
         for k in MESH_FIELDS:
@@ -119,7 +126,7 @@ class RichGPU:
             keep.append(a)
             ptrs.append(_p(a))
         ms = MeshS(mesh.n_cells, mesh.n_faces, mesh.n_ghosts, *ptrs)
-        self._check(self._lib.rg_set_mesh(self._h, ctypes.byref(ms)))
+        self._check(fn(self._h, ctypes.byref(ms)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -155,6 +162,31 @@ class RichGPU:
         self._wkeep = self._rows(w, self.F, 2)
         self._check(self._lib.rg_set_face_velocity(self._h, _p(self._wkeep[0]),
                                                    _p(self._wkeep[1])))
+
+    def moving_begin(self, chi, eta, box, velocities=True):
+        """Stage 1 of a moving-mesh Heun step: (dt, vx, vy) with the point
+        velocities (None, None without `velocities`)."""
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        if b.shape != (4,):
+            raise ValueError("box is (x0, x1, y0, y1)")
+        dt = ctypes.c_double()
+        vx = np.empty(self.N) if velocities else None
+        vy = np.empty(self.N) if velocities else None
+        self._check(self._lib.rg_moving_begin(self._h, float(chi), float(eta), _p(b),
+                                              ctypes.byref(dt), _p(vx), _p(vy)))
+        return dt.value, vx, vy
+
+    def replace_mesh(self, mesh):
+        """The rebuilt mesh of the moved points; the device state is kept."""
+        if mesh.n_cells != self.N:
+            raise ValueError("replace_mesh keeps the cell count")
+        self._upload(self._lib.rg_replace_mesh, mesh)
+        self.mesh = mesh
+        self.F = mesh.n_faces
+
+    def moving_end(self):
+        """Stage 2 of the moving-mesh step on the replaced mesh."""
+        self._check(self._lib.rg_moving_end(self._h))
 
     def set_cells(self, cells):
         c = self._rows(cells, self.N)

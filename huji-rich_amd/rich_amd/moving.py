@@ -3,13 +3,19 @@ hdsim::TimeAdvance2Heun (hdsim2d.cpp:323-449) with Lagrangian point motion
 and RoundCells (round_cells.cpp:63-134), StationaryBox face velocities and
 a host Voronoi rebuild every step.
 
-The point motion and the mesh rebuild run on the host, as in RICH; the
-plugins (CFL, reconstruction and fluxes, extensive and cell updates) run on
-the backend.  `GpuBackend` calls the HIP library through its C ABI; any
-object with the same five methods can stand in for it (the tests pass the
-CPU oracle to check the GPU run step by step).  Cells keep the order of
-their points: the Hilbert re-sort that hdsim applies every 25 cycles only
-permutes cells and is not done."""
+`MovingMesh` runs the point motion and the mesh rebuild on the host, as in
+RICH, and the plugins (CFL, reconstruction and fluxes, extensive and cell
+updates) on a backend.  `GpuBackend` calls the HIP library through its C
+ABI; any object with the same five methods can stand in for it (the tests
+pass the CPU oracle to check the GPU run step by step).
+
+`DeviceMovingMesh` keeps cells, extensives and point velocities on the GPU:
+point and face velocities, the overshoot fix, both CFL passes and both
+stages run there (rg_moving_begin / rg_moving_end); the host moves the
+points and rebuilds the mesh (rg_replace_mesh).  Both give the same values.
+
+Cells keep the order of their points: the Hilbert re-sort that hdsim
+applies every 25 cycles only permutes cells and is not done."""
 import time
 
 import numpy as np
@@ -176,3 +182,46 @@ class MovingMesh:
         self.cells = self._plug(b.update_cells, ext)
         self.time += dt
         return dt
+
+
+class DeviceMovingMesh:
+    """A moving-mesh run with the state on the GPU; `step()` advances one
+    TimeAdvance2Heun step, `cells()` reads the cells back."""
+
+    def __init__(self, x, y, cells, box, gamma=5.0 / 3.0, cfl=0.3, chi=0.15, eta=0.02):
+        self.x = np.array(x, dtype=np.float64)
+        self.y = np.array(y, dtype=np.float64)
+        self.box, self.chi, self.eta = tuple(float(v) for v in box), chi, eta
+        self.mesh = rmesh.points(self.x, self.y, box)
+        self.g = RichGPU(self.mesh, gamma=gamma, cfl=cfl, order=2)
+        self.g.set_cells(np.asarray(cells, dtype=np.float64))
+        self.g.init_extensives()
+        self.time = 0.0
+        self.timing = {"mesh_s": 0.0, "device_s": 0.0}
+
+    def step(self):
+        T = self.timing
+        t = time.perf_counter()
+        dt, vx, vy = self.g.moving_begin(self.chi, self.eta, self.box)
+        T["device_s"] += time.perf_counter() - t
+        # MoveMeshPoints (hydrodynamics_2d.cpp:218-247) and the rebuild
+        t = time.perf_counter()
+        self.x = self.x + dt * vx
+        self.y = self.y + dt * vy
+        self.mesh = rmesh.points(self.x, self.y, self.box)
+        T["mesh_s"] += time.perf_counter() - t
+        t = time.perf_counter()
+        self.g.replace_mesh(self.mesh)
+        self.g.moving_end()
+        T["device_s"] += time.perf_counter() - t
+        self.time += dt
+        return dt
+
+    def cells(self):
+        return self.g.get_cells()
+
+    def extensives(self):
+        return self.g.get_extensives()
+
+    def close(self):
+        self.g.close()

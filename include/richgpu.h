@@ -113,6 +113,27 @@ int rg_set_mesh(rg_ctx* ctx, const rg_mesh* mesh);
 /* Face velocities [F] as hdsim passes edge_velocities; NULL,NULL = all zero
  * (StationaryBox/PeriodicEdgeVelocities on a static mesh). */
 int rg_set_face_velocity(rg_ctx* ctx, const double* wx, const double* wy);
+/* moving mesh (rigid box, order 2, RG_FP64) ---------------------------
+ * One hdsim::TimeAdvance2Heun step with Lagrangian + RoundCells points and
+ * StationaryBox edges is rg_moving_begin, the host moving the points by
+ * dt * velocity and rebuilding the Voronoi mesh, rg_replace_mesh, then
+ * rg_moving_end; cells, extensives and point velocities stay on the device.
+ *
+ * Point velocities (point_motion_, hdsim2d.cpp:325; round_cells.cpp:78-134
+ * with chi, eta), edge velocities, the SimpleCFL dt, ApplyFix of a rigid box
+ * [x0, x1, y0, y1] (round_cells.cpp:14-61), edge velocities and dt again,
+ * then stage 1 on the current mesh (hdsim2d.cpp:325-371).  Returns dt and,
+ * where pvx/pvy are non-NULL, the point velocities [N]. */
+int rg_moving_begin(rg_ctx* ctx, double chi, double eta, const double* box, double* dt_out,
+                    double* pvx, double* pvy);
+/* The tessellation rebuilt on the moved points (MoveMeshPoints,
+ * hdsim2d.cpp:390; same cell count and order): replaces the mesh and keeps
+ * cells, extensives, mid extensives and point velocities. */
+int rg_replace_mesh(rg_ctx* ctx, const rg_mesh* mesh);
+/* Stage 2 on the replaced mesh: mid cells, edge velocities from the kept
+ * point velocities, fluxes, the update, average_extensive and the cell
+ * update (hdsim2d.cpp:401-441). */
+int rg_moving_end(rg_ctx* ctx);
 /* cells_ (ComputationalCell density, pressure, velocity), [N] each. */
 int rg_set_cells(rg_ctx* ctx, const double* d, const double* p, const double* vx,
                  const double* vy);

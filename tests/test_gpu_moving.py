@@ -53,3 +53,43 @@ def test_moving_sedov_gpu_equals_oracle(n):
           f"(initially {m.n_faces}); t = {runs[0].time:.4e}")
     assert moved > 0
     runs[0].b.close()
+
+
+@pytest.mark.parametrize("n", [20, 32])
+def test_moving_device_resident_equals_oracle(n):
+    """rg_moving_begin / rg_replace_mesh / rg_moving_end (point and face
+    velocities, the overshoot fix and both stages on the device) against the
+    host-driven run over the oracle: same points, dt and state every step."""
+    from rich_amd.moving import DeviceMovingMesh, MovingMesh
+    m = cases.sedov_mesh(n)
+    c = cases.sedov_cells(m)
+    N = m.n_cells
+    box = (-1.0, 1.0, -1.0, 1.0)
+    dev = DeviceMovingMesh(m.pt_x[:N], m.pt_y[:N], c, box)
+    ref = MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, OracleBackend())
+    for step in range(8):
+        dt_dev, dt_ref = dev.step(), ref.step()
+        print(f"n={n} step={step} dt={dt_dev:.6e}", flush=True)
+        assert dt_dev == dt_ref, step
+        np.testing.assert_array_equal(dev.x, ref.x)
+        np.testing.assert_array_equal(dev.y, ref.y)
+        np.testing.assert_array_equal(dev.cells(), ref.cells)
+        np.testing.assert_array_equal(dev.extensives(), ref.ext)
+    assert dev.mesh.n_faces != m.n_faces or np.any(dev.x != m.pt_x[:N])
+    dev.close()
+
+
+def test_moving_device_rejects_bad_use():
+    """rg_moving_end needs a preceding rg_moving_begin; rg_replace_mesh keeps
+    the cell count."""
+    from rich_amd.gpu import RichGPU, RichGPUError
+    m = cases.sedov_mesh(16)
+    g = RichGPU(m, order=2)
+    g.set_cells(cases.sedov_cells(m))
+    g.init_extensives()
+    with pytest.raises(RichGPUError):
+        g.moving_end()
+    other = cases.sedov_mesh(12)
+    with pytest.raises(ValueError):
+        g.replace_mesh(other)
+    g.close()
