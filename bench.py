@@ -136,12 +136,16 @@ def moving_leg(args):
            "cells": N, "steps": steps, "sim_time": dev.time,
            "s_per_step": el / steps, "value": N * steps / el / 1e6, "unit": "Mcell-updates/s",
            "host_mesh_s_per_step": T["mesh_s"] / steps,
-           "device_s_per_step": T["device_s"] / steps,
+           "begin_s_per_step": T["begin_s"] / steps,
+           "replace_mesh_s_per_step": T["replace_s"] / steps,
+           "end_s_per_step": T["end_s"] / steps,
+           "mesh_builds_repairs": list(dev.mesher.counts()),
            "mass_drift": float(x1[0].sum() / x0[0].sum() - 1.0),
            "energy_drift": float(x1[1].sum() / x0[1].sum() - 1.0),
            "setup_s": setup,
-           "note": "device_s includes the mesh upload of rg_replace_mesh and the point "
-                   "velocities copied back for the host rebuild"}
+           "note": "begin: point velocities, dt and stage 1 on the device, point velocities "
+                   "copied back; replace: the rebuilt mesh's tables built and uploaded; end: "
+                   "stage 2 on the device"}
     dev.close()
     run = MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, GpuBackend())
     run.timing = {k: 0.0 for k in run.timing}

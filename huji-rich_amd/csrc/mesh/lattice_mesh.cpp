@@ -8,6 +8,7 @@
 // copies for periodic boxes) extend the lattice by one row/column on each
 // side, which is all a real cell's Voronoi polygon depends on.
 #include "../../../include/richmesh.h"
+#include "delaunay.hpp"
 
 #include <cmath>
 #include <cstdint>
@@ -103,13 +104,19 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
                  int flip_seed, const std::vector<int>& ghost_wall, double x0, double x1,
                  double y0, double y1) {
   const int32_t N = m->N, G = m->G;
-  std::sort(edges.begin(), edges.end(), [](const EdgeRef& a, const EdgeRef& b) {
+  auto before = [](const EdgeRef& a, const EdgeRef& b) {
     return a.tlo != b.tlo ? a.tlo < b.tlo : a.k < b.k;
-  });
+  };
+  if (!std::is_sorted(edges.begin(), edges.end(), before))
+    std::sort(edges.begin(), edges.end(), before);
   // 6. faces, dropping near-zero-length ones (VoronoiMesh.cpp:287-288)
   uint64_t fs = static_cast<uint64_t>(flip_seed) * 0x9E3779B97F4A7C15ull + 12345u;
-  std::vector<std::vector<int32_t>> cell_faces(static_cast<size_t>(N));
-  for (auto& cf : cell_faces) cf.reserve(8);
+  m->ff.reserve(edges.size());
+  m->fs.reserve(edges.size());
+  m->v1x.reserve(edges.size());
+  m->v1y.reserve(edges.size());
+  m->v2x.reserve(edges.size());
+  m->v2y.reserve(edges.size());
   for (const EdgeRef& e : edges) {
     const Tri &tl = tris[e.tlo], &th = tris[e.thi];
     const double lx = th.cc.x - tl.cc.x, ly = th.cc.y - tl.cc.y;
@@ -124,17 +131,24 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
     m->v1y.push_back(tl.cc.y);
     m->v2x.push_back(th.cc.x);
     m->v2y.push_back(th.cc.y);
-    if (first < N) cell_faces[first].push_back(f);
-    if (second < N) cell_faces[second].push_back(f);
+    (void)f;
   }
   m->F = static_cast<int32_t>(m->ff.size());
-  m->off.resize(static_cast<size_t>(N) + 1);
-  m->off[0] = 0;
-  for (int32_t c = 0; c < N; ++c)
-    m->off[c + 1] = m->off[c] + static_cast<int32_t>(cell_faces[c].size());
+  // CSR: each cell's faces in ascending face order
+  m->off.assign(static_cast<size_t>(N) + 1, 0);
+  for (int32_t f = 0; f < m->F; ++f) {
+    if (m->ff[f] < N) ++m->off[m->ff[f] + 1];
+    if (m->fs[f] < N) ++m->off[m->fs[f] + 1];
+  }
+  for (int32_t c = 0; c < N; ++c) m->off[c + 1] += m->off[c];
   m->csr.resize(static_cast<size_t>(m->off[N]));
-  for (int32_t c = 0; c < N; ++c)
-    std::copy(cell_faces[c].begin(), cell_faces[c].end(), m->csr.begin() + m->off[c]);
+  {
+    std::vector<int32_t> fill(m->off.begin(), m->off.end() - 1);
+    for (int32_t f = 0; f < m->F; ++f) {
+      if (m->ff[f] < N) m->csr[fill[m->ff[f]]++] = f;
+      if (m->fs[f] < N) m->csr[fill[m->fs[f]]++] = f;
+    }
+  }
 
   // 7. cell geometry, in each cell's face order
   m->vol.resize(N);
@@ -428,60 +442,100 @@ extern "C" rm_mesh* rm_lattice_build(const rm_lattice_params* prm) {
   return m;
 }
 
-extern "C" rm_mesh* rm_points_build(const rm_points_params* prm) {
-  if (!prm || prm->n_real < 3 || prm->n_ghost < 0 || prm->n_tri < 1 || !prm->x || !prm->y ||
-      !prm->tri || (prm->n_ghost > 0 && (!prm->ghost_orig || !prm->ghost_wall)) ||
-      !(prm->x1 > prm->x0) || !(prm->y1 > prm->y0))
-    return nullptr;
-  const int32_t N = prm->n_real, G = prm->n_ghost, NP = N + G;
+namespace {
+
+// Faces and geometry of a rigid-box mesh from Delaunay triangles of the
+// real points [0, N) and their wall reflections [N, N+G).  With `half`
+// (the opposite half-edge of each triangle edge, -1 where none) triangles
+// must be counter-clockwise; without it they may have either orientation
+// and shared edges are matched by point pair.
+rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
+                     const int32_t* ghost_orig, const int32_t* gwall, int32_t n_tri,
+                     const int32_t* tri, const int32_t* half, double x0, double x1, double y0,
+                     double y1) {
+  const int32_t NP = N + G;
   rm_mesh* m = new rm_mesh();
   m->N = N;
   m->G = G;
   m->bc = RM_BC_RIGID;
-  m->px.assign(prm->x, prm->x + NP);
-  m->py.assign(prm->y, prm->y + NP);
+  m->px.assign(x, x + NP);
+  m->py.assign(y, y + NP);
   std::vector<int> ghost_wall(static_cast<size_t>(G));
   for (int32_t g = 0; g < G; ++g) {
-    if (prm->ghost_orig[g] < 0 || prm->ghost_orig[g] >= N || prm->ghost_wall[g] < 0 ||
-        prm->ghost_wall[g] > 3) {
+    if (ghost_orig[g] < 0 || ghost_orig[g] >= N || gwall[g] < 0 || gwall[g] > 3) {
       delete m;
       return nullptr;
     }
-    m->gorig.push_back(prm->ghost_orig[g]);
-    ghost_wall[g] = prm->ghost_wall[g];
+    m->gorig.push_back(ghost_orig[g]);
+    ghost_wall[g] = gwall[g];
   }
   auto P = [&](int32_t id) { return V2{m->px[id], m->py[id]}; };
-  // triangles counter-clockwise, numbered in Hilbert order of their centroids
-  double bx0 = m->px[0], bx1 = bx0, by0 = m->py[0], by1 = by0;
-  for (int32_t i = 1; i < NP; ++i) {
-    bx0 = std::min(bx0, m->px[i]);
-    bx1 = std::max(bx1, m->px[i]);
-    by0 = std::min(by0, m->py[i]);
-    by1 = std::max(by1, m->py[i]);
-  }
-  const uint64_t hn = 1u << 16;
-  std::vector<std::pair<uint64_t, int32_t>> order(static_cast<size_t>(prm->n_tri));
-  for (int32_t t = 0; t < prm->n_tri; ++t) {
-    const int32_t* q = prm->tri + 3 * static_cast<size_t>(t);
+  // triangles numbered by their point ids (smallest first, then the next
+  // two counter-clockwise), so they, and the faces after them, follow the
+  // cell order and depend only on the set of triangles
+  std::vector<int32_t> cnt(static_cast<size_t>(NP) + 1, 0);
+  for (int32_t t = 0; t < n_tri; ++t) {
+    const int32_t* q = tri + 3 * static_cast<size_t>(t);
     for (int k = 0; k < 3; ++k)
       if (q[k] < 0 || q[k] >= NP) {
         delete m;
         return nullptr;
       }
-    const double cx = (m->px[q[0]] + m->px[q[1]] + m->px[q[2]]) / 3.0;
-    const double cy = (m->py[q[0]] + m->py[q[1]] + m->py[q[2]]) / 3.0;
-    const uint64_t ix = static_cast<uint64_t>((cx - bx0) / (bx1 - bx0 + 1e-300) * (hn - 1));
-    const uint64_t iy = static_cast<uint64_t>((cy - by0) / (by1 - by0 + 1e-300) * (hn - 1));
-    order[t] = {hilbert_d(hn, ix, iy), t};
+    ++cnt[std::min(q[0], std::min(q[1], q[2])) + 1];
   }
-  std::sort(order.begin(), order.end());
+  for (int32_t i = 0; i < NP; ++i) cnt[i + 1] += cnt[i];
+  std::vector<std::pair<int32_t, int32_t>> order(static_cast<size_t>(n_tri));
+  for (int32_t t = 0; t < n_tri; ++t) {
+    const int32_t* q = tri + 3 * static_cast<size_t>(t);
+    const int32_t lo = std::min(q[0], std::min(q[1], q[2]));
+    order[cnt[lo]++] = {lo, t};
+  }
+  {
+    // within one smallest id: by the ids that follow it (orientation as
+    // given, which is counter-clockwise where `half` is)
+    auto next2 = [&](int32_t t) {
+      const int32_t* q = tri + 3 * static_cast<size_t>(t);
+      const int k = q[0] <= q[1] && q[0] <= q[2] ? 0 : (q[1] <= q[2] ? 1 : 2);
+      return std::make_pair(q[(k + 1) % 3], q[(k + 2) % 3]);
+    };
+    size_t b = 0;
+    while (b < order.size()) {
+      size_t e = b + 1;
+      while (e < order.size() && order[e].first == order[b].first) ++e;
+      std::sort(order.begin() + b, order.begin() + e,
+                [&](const std::pair<int32_t, int32_t>& u, const std::pair<int32_t, int32_t>& v) {
+                  return next2(u.second) < next2(v.second);
+                });
+      b = e;
+    }
+  }
   std::vector<Tri> tris;
   tris.reserve(order.size());
-  for (const auto& o : order) {
-    const int32_t* q = prm->tri + 3 * static_cast<size_t>(o.second);
+  std::vector<int8_t> rot(half ? order.size() : 0);
+  for (size_t r = 0; r < order.size(); ++r) {
+    const int32_t* q = tri + 3 * static_cast<size_t>(order[r].second);
     int32_t a = q[0], b = q[1], c = q[2];
-    const V2 B0 = sub(P(b), P(a)), C0 = sub(P(c), P(a));
-    if (B0.x * C0.y - B0.y * C0.x < 0) std::swap(b, c);
+    if (!half) {
+      const V2 B0 = sub(P(b), P(a)), C0 = sub(P(c), P(a));
+      if (B0.x * C0.y - B0.y * C0.x < 0) std::swap(b, c);
+    }
+    // start at the smallest id, so the triangle (its circumcentre's
+    // round-off, its edge slots) does not depend on the triangulator
+    int k0 = 0;
+    if (b < a && b < c) k0 = 1;
+    if (c < a && c < b) k0 = 2;
+    if (k0 == 1) {
+      const int32_t t0 = a;
+      a = b;
+      b = c;
+      c = t0;
+    } else if (k0 == 2) {
+      const int32_t t0 = c;
+      c = b;
+      b = a;
+      a = t0;
+    }
+    if (half) rot[r] = static_cast<int8_t>(k0);
     Tri t;
     t.p[0] = a;
     t.p[1] = b;
@@ -497,29 +551,249 @@ extern "C" rm_mesh* rm_points_build(const rm_points_params* prm) {
   }
   // shared edges: the first triangle (in the order above) to list an edge
   // owns it; the second one closes it
-  std::unordered_map<uint64_t, std::pair<int32_t, int32_t>> open;  // key -> (triangle, slot)
-  open.reserve(tris.size() * 2);
   std::vector<EdgeRef> edges;
   edges.reserve(tris.size() * 3 / 2);
-  for (int32_t t = 0; t < static_cast<int32_t>(tris.size()); ++t)
-    for (int k = 0; k < 3; ++k) {
-      const int32_t u = tris[t].p[k], w = tris[t].p[(k + 1) % 3];
-      const uint64_t key = (static_cast<uint64_t>(std::min(u, w)) << 32) |
-                           static_cast<uint32_t>(std::max(u, w));
-      auto it = open.find(key);
-      if (it == open.end()) {
-        open.emplace(key, std::make_pair(t, k));
-        continue;
+  if (half) {
+    std::vector<int32_t> rank(static_cast<size_t>(n_tri));
+    for (int32_t r = 0; r < n_tri; ++r) rank[order[r].second] = r;
+    for (int32_t r = 0; r < n_tri; ++r) {
+      const int32_t t0 = order[r].second;
+      for (int k = 0; k < 3; ++k) {
+        const int32_t h = half[3 * static_cast<size_t>(t0) + (k + rot[r]) % 3];
+        if (h < 0) continue;
+        const int32_t r2 = rank[h / 3];
+        if (r2 <= r) continue;
+        const int32_t a = tris[r].p[k], b = tris[r].p[(k + 1) % 3];
+        if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
+        edges.push_back(EdgeRef{r, r2, a, b, k});
       }
-      const int32_t tlo = it->second.first, klo = it->second.second;
-      open.erase(it);
-      const int32_t a = tris[tlo].p[klo], b = tris[tlo].p[(klo + 1) % 3];
-      if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
-      edges.push_back(EdgeRef{tlo, t, a, b, klo});
     }
-  finish_mesh(m, tris, edges, 0, ghost_wall, prm->x0, prm->x1, prm->y0, prm->y1);
+  } else {
+    std::unordered_map<uint64_t, std::pair<int32_t, int32_t>> open;  // key -> (triangle, slot)
+    open.reserve(tris.size() * 2);
+    for (int32_t t = 0; t < static_cast<int32_t>(tris.size()); ++t)
+      for (int k = 0; k < 3; ++k) {
+        const int32_t u = tris[t].p[k], w = tris[t].p[(k + 1) % 3];
+        const uint64_t key = (static_cast<uint64_t>(std::min(u, w)) << 32) |
+                             static_cast<uint32_t>(std::max(u, w));
+        auto it = open.find(key);
+        if (it == open.end()) {
+          open.emplace(key, std::make_pair(t, k));
+          continue;
+        }
+        const int32_t tlo = it->second.first, klo = it->second.second;
+        open.erase(it);
+        const int32_t a = tris[tlo].p[klo], b = tris[tlo].p[(klo + 1) % 3];
+        if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
+        edges.push_back(EdgeRef{tlo, t, a, b, klo});
+      }
+  }
+  finish_mesh(m, tris, edges, 0, ghost_wall, x0, x1, y0, y1);
   return m;
 }
+
+}  // namespace
+
+extern "C" rm_mesh* rm_points_build(const rm_points_params* prm) {
+  if (!prm || prm->n_real < 3 || prm->n_ghost < 0 || prm->n_tri < 1 || !prm->x || !prm->y ||
+      !prm->tri || (prm->n_ghost > 0 && (!prm->ghost_orig || !prm->ghost_wall)) ||
+      !(prm->x1 > prm->x0) || !(prm->y1 > prm->y0))
+    return nullptr;
+  return points_mesh(prm->n_real, prm->n_ghost, prm->x, prm->y, prm->ghost_orig, prm->ghost_wall,
+                     prm->n_tri, prm->tri, nullptr, prm->x0, prm->x1, prm->y0, prm->y1);
+}
+
+namespace {
+
+// Reflection of point (px, py) across wall w of the box (0 right, 1 up,
+// 2 left, 3 down).
+inline void reflect_point(int w, double px, double py, double x0, double x1, double y0, double y1,
+                          double& gx, double& gy) {
+  gx = w == 0 ? px + 2 * (x1 - px) : w == 2 ? px + 2 * (x0 - px) : px + 0.0;
+  gy = w == 1 ? py + 2 * (y1 - py) : w == 3 ? py + 2 * (y0 - py) : py + 0.0;
+}
+
+inline bool near_wall(int w, double px, double py, double x0, double x1, double y0, double y1,
+                      double mg) {
+  return w == 0 ? x1 - px < mg : w == 1 ? y1 - py < mg : w == 2 ? px - x0 < mg : py - y0 < mg;
+}
+
+// The points followed by the reflections across each wall of the points
+// within the margin of it, wall by wall, points in index order.
+void box_ghosts(int32_t N, const double* x, const double* y, double x0, double x1, double y0,
+                double y1, double mg, std::vector<double>& X, std::vector<double>& Y,
+                std::vector<int32_t>& go, std::vector<int32_t>& gw) {
+  X.assign(x, x + N);
+  Y.assign(y, y + N);
+  go.clear();
+  gw.clear();
+  for (int w = 0; w < 4; ++w)
+    for (int32_t i = 0; i < N; ++i) {
+      if (!near_wall(w, x[i], y[i], x0, x1, y0, y1, mg)) continue;
+      double gx, gy;
+      reflect_point(w, x[i], y[i], x0, x1, y0, y1, gx, gy);
+      X.push_back(gx);
+      Y.push_back(gy);
+      go.push_back(i);
+      gw.push_back(w);
+    }
+}
+
+// The mesh from a triangulation of points and ghosts: the triangles with a
+// real vertex and the ghosts they use, ghosts renumbered in their order.
+rm_mesh* mesh_of_triangulation(int32_t N, const std::vector<double>& X,
+                               const std::vector<double>& Y, const std::vector<int32_t>& go,
+                               const std::vector<int32_t>& gw, const std::vector<int32_t>& tri,
+                               const std::vector<int32_t>& half, double x0, double x1, double y0,
+                               double y1) {
+  const int32_t T = static_cast<int32_t>(tri.size() / 3);
+  const int32_t G0 = static_cast<int32_t>(go.size());
+  std::vector<int32_t> keep(static_cast<size_t>(T), -1), gid(static_cast<size_t>(G0), -1);
+  int32_t nt = 0;
+  for (int32_t t = 0; t < T; ++t) {
+    const int32_t* q = &tri[3 * static_cast<size_t>(t)];
+    if (q[0] >= N && q[1] >= N && q[2] >= N) continue;
+    keep[t] = nt++;
+    for (int k = 0; k < 3; ++k) {
+      if (q[k] >= N + G0) return nullptr;  // a real point on the hull of its reflections
+      if (q[k] >= N) gid[q[k] - N] = 0;
+    }
+  }
+  std::vector<double> px(X.begin(), X.begin() + N), py(Y.begin(), Y.begin() + N);
+  std::vector<int32_t> gorig, gwall;
+  int32_t G = 0;
+  for (int32_t g = 0; g < G0; ++g)
+    if (gid[g] == 0) {
+      gid[g] = N + G++;
+      px.push_back(X[N + g]);
+      py.push_back(Y[N + g]);
+      gorig.push_back(go[g]);
+      gwall.push_back(gw[g]);
+    }
+  std::vector<int32_t> t2(3 * static_cast<size_t>(nt)), h2(3 * static_cast<size_t>(nt));
+  for (int32_t t = 0; t < T; ++t) {
+    if (keep[t] < 0) continue;
+    for (int k = 0; k < 3; ++k) {
+      const size_t e = 3 * static_cast<size_t>(t) + k, e2 = 3 * static_cast<size_t>(keep[t]) + k;
+      const int32_t v = tri[e];
+      t2[e2] = v < N ? v : gid[v - N];
+      const int32_t h = half[e];
+      h2[e2] = (h >= 0 && keep[h / 3] >= 0) ? 3 * keep[h / 3] + h % 3 : -1;
+    }
+  }
+  return points_mesh(N, G, px.data(), py.data(), gorig.data(), gwall.data(), nt, t2.data(),
+                     h2.data(), x0, x1, y0, y1);
+}
+
+bool box_ok(const rm_box_points_params* p) {
+  return p && p->n >= 3 && p->x1 > p->x0 && p->y1 > p->y0 && p->margin > 0;
+}
+
+}  // namespace
+
+extern "C" rm_mesh* rm_box_points_build(const rm_box_points_params* prm) {
+  if (!box_ok(prm) || !prm->x || !prm->y) return nullptr;
+  std::vector<double> X, Y;
+  std::vector<int32_t> go, gw, tri, half;
+  box_ghosts(prm->n, prm->x, prm->y, prm->x0, prm->x1, prm->y0, prm->y1, prm->margin, X, Y, go,
+             gw);
+  if (rmd::triangulate(static_cast<int32_t>(X.size()), X.data(), Y.data(), tri, half) != 0)
+    return nullptr;
+  return mesh_of_triangulation(prm->n, X, Y, go, gw, tri, half, prm->x0, prm->x1, prm->y0,
+                               prm->y1);
+}
+
+struct rm_mover {
+  int32_t N = 0;
+  double x0 = 0, x1 = 0, y0 = 0, y1 = 0, margin = 0;
+  std::vector<int32_t> go, gw;    // ghosts of the last full build
+  std::vector<uint8_t> has;       // [4 N]: point i has a ghost across wall w
+  std::vector<double> X, Y;       // points, ghosts and a fixed frame
+  std::vector<int32_t> tri, half; // their triangulation
+  int32_t builds = 0, repairs = 0;
+};
+
+extern "C" rm_mover* rm_mover_create(const rm_box_points_params* prm) {
+  if (!box_ok(prm)) return nullptr;
+  rm_mover* mv = new rm_mover();
+  mv->N = prm->n;
+  mv->x0 = prm->x0;
+  mv->x1 = prm->x1;
+  mv->y0 = prm->y0;
+  mv->y1 = prm->y1;
+  mv->margin = prm->margin;
+  return mv;
+}
+
+extern "C" rm_mesh* rm_mover_mesh(rm_mover* mv, const double* x, const double* y) {
+  if (!mv || !x || !y) return nullptr;
+  const int32_t N = mv->N;
+  for (int32_t i = 0; i < N; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i])) return nullptr;
+  bool ok = !mv->tri.empty();
+  // every point near a wall still has its reflection
+  for (int w = 0; ok && w < 4; ++w)
+    for (int32_t i = 0; i < N; ++i)
+      if (!mv->has[static_cast<size_t>(w) * N + i] &&
+          near_wall(w, x[i], y[i], mv->x0, mv->x1, mv->y0, mv->y1, mv->margin)) {
+        ok = false;
+        break;
+      }
+  if (ok) {  // the frame stays where it is
+    std::copy(x, x + N, mv->X.begin());
+    std::copy(y, y + N, mv->Y.begin());
+    for (size_t g = 0; g < mv->go.size(); ++g)
+      reflect_point(mv->gw[g], x[mv->go[g]], y[mv->go[g]], mv->x0, mv->x1, mv->y0, mv->y1,
+                    mv->X[N + g], mv->Y[N + g]);
+    ok = rmd::repair(static_cast<int32_t>(mv->X.size()), mv->X.data(), mv->Y.data(), mv->tri,
+                     mv->half) == 0;
+    if (ok) ++mv->repairs;
+  }
+  if (!ok) {
+    box_ghosts(N, x, y, mv->x0, mv->x1, mv->y0, mv->y1, mv->margin, mv->X, mv->Y, mv->go, mv->gw);
+    // a fixed frame of points around the ghosts (a square ring two mean
+    // spacings beyond them, one spacing apart): the convex hull never
+    // changes as points move, and the outermost ghosts are joined to the
+    // frame by well-shaped triangles instead of nearly flat ones
+    const double h = std::sqrt((mv->x1 - mv->x0) * (mv->y1 - mv->y0) / N);
+    const double off = mv->margin + 2.0 * h;
+    const double fx0 = mv->x0 - off, fx1 = mv->x1 + off, fy0 = mv->y0 - off, fy1 = mv->y1 + off;
+    const int nx = static_cast<int>(std::ceil((fx1 - fx0) / h));
+    const int ny = static_cast<int>(std::ceil((fy1 - fy0) / h));
+    for (int i = 0; i < nx; ++i) {  // bottom and top, corners included
+      const double fx = fx0 + (fx1 - fx0) * i / nx;
+      mv->X.push_back(fx);
+      mv->Y.push_back(fy0);
+      mv->X.push_back(fx1 - (fx1 - fx0) * i / nx);
+      mv->Y.push_back(fy1);
+    }
+    for (int j = 0; j < ny; ++j) {  // right and left
+      mv->X.push_back(fx1);
+      mv->Y.push_back(fy0 + (fy1 - fy0) * j / ny);
+      mv->X.push_back(fx0);
+      mv->Y.push_back(fy1 - (fy1 - fy0) * j / ny);
+    }
+    mv->has.assign(4 * static_cast<size_t>(N), 0);
+    for (size_t g = 0; g < mv->go.size(); ++g)
+      mv->has[static_cast<size_t>(mv->gw[g]) * N + mv->go[g]] = 1;
+    if (rmd::triangulate(static_cast<int32_t>(mv->X.size()), mv->X.data(), mv->Y.data(), mv->tri,
+                         mv->half) != 0) {
+      mv->tri.clear();
+      mv->half.clear();
+      return nullptr;
+    }
+    ++mv->builds;
+  }
+  return mesh_of_triangulation(N, mv->X, mv->Y, mv->go, mv->gw, mv->tri, mv->half, mv->x0,
+                               mv->x1, mv->y0, mv->y1);
+}
+
+extern "C" void rm_mover_counts(const rm_mover* mv, int32_t* builds, int32_t* repairs) {
+  if (builds) *builds = mv ? mv->builds : 0;
+  if (repairs) *repairs = mv ? mv->repairs : 0;
+}
+
+extern "C" void rm_mover_free(rm_mover* mv) { delete mv; }
 
 extern "C" void rm_free(rm_mesh* m) { delete m; }
 

@@ -29,6 +29,13 @@ class _PointsParams(ctypes.Structure):
                 ("y0", ctypes.c_double), ("y1", ctypes.c_double)]
 
 
+class _BoxPointsParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("x0", ctypes.c_double), ("x1", ctypes.c_double),
+                ("y0", ctypes.c_double), ("y1", ctypes.c_double),
+                ("margin", ctypes.c_double)]
+
+
 def _lib():
     lib = _ffi.load("librichmesh")
     if not getattr(lib, "_typed", False):
@@ -40,6 +47,17 @@ def _lib():
         lib.rm_sizes.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int32)] * 4
         lib.rm_export.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 17
         lib.rm_face_velocity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        lib.rm_mover_create.restype = ctypes.c_void_p
+        lib.rm_mover_create.argtypes = [ctypes.POINTER(_BoxPointsParams)]
+        lib.rm_mover_mesh.restype = ctypes.c_void_p
+        lib.rm_mover_mesh.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.rm_mover_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_int32)]
+        lib.rm_mover_free.argtypes = [ctypes.c_void_p]
+        lib.rm_box_points_build.restype = ctypes.c_void_p
+        lib.rm_box_points_build.argtypes = [ctypes.POINTER(_BoxPointsParams)]
+        lib.rm_delaunay.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         lib._typed = True
     return lib
 
@@ -112,21 +130,92 @@ def lattice(nx, ny, box=(0.0, 1.0, 0.0, 1.0), jitter=1e-3, seed=0, bc=RIGID, fli
     return _export(h, bc, lattice_ij=True)
 
 
+def delaunay(x, y):
+    """(T, 3) counter-clockwise Delaunay triangles of the points: rm_delaunay,
+    or Qhull (scipy.spatial.Delaunay) where rm_delaunay reports an input it
+    cannot triangulate (duplicate points, a point its sweep cannot place)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    n = len(x)
+    tri = np.empty((2 * n, 3), dtype=np.int32)
+    nt = ctypes.c_int32()
+    rc = _lib().rm_delaunay(n, _ffi.ptr(x), _ffi.ptr(y), _ffi.ptr(tri), 2 * n, ctypes.byref(nt))
+    if rc == 0:
+        return tri[:nt.value]
+    from scipy.spatial import Delaunay
+    return Delaunay(np.column_stack([x, y])).simplices.astype(np.int32)
+
+
 def points(x, y, box, margin=None):
     """Voronoi mesh of arbitrary points in the rigid-wall box (x0, x1, y0,
     y1), e.g. after the points moved.  Ghosts are the reflections across each
     wall of the points within `margin` of it (default: 3 mean spacings), as
-    the lattice builder reflects its outer rows; the Delaunay triangulation of
-    points and ghosts comes from Qhull (scipy.spatial.Delaunay) and
-    rm_points_build forms faces and geometry by the lattice builder's rules.
-    Cells keep the order of the points."""
-    from scipy.spatial import Delaunay
+    the lattice builder reflects its outer rows; rm_box_points_build
+    triangulates points and ghosts and forms faces and geometry by the
+    lattice builder's rules.  Where it cannot triangulate the points, Qhull
+    (scipy.spatial.Delaunay) does and rm_points_build forms the mesh.  Cells
+    keep the order of the points."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     N = len(x)
     x0, x1, y0, y1 = box
     if margin is None:
         margin = 3.0 * np.sqrt((x1 - x0) * (y1 - y0) / N)
+    bp = _BoxPointsParams(N, _ffi.ptr(x), _ffi.ptr(y), x0, x1, y0, y1, margin)
+    h = _lib().rm_box_points_build(ctypes.byref(bp))
+    if h:
+        return _export(h, RIGID, lattice_ij=False)
+    return _points_qhull(x, y, box, margin)
+
+
+class BoxMesher:
+    """Meshes of n moving points in a rigid-wall box, one per step
+    (rm_mover_mesh): the previous step's triangulation repaired by edge flips
+    where it can be, built afresh otherwise; the same meshes as `points`."""
+
+    def __init__(self, n, box, margin=None):
+        x0, x1, y0, y1 = (float(v) for v in box)
+        if margin is None:
+            margin = 3.0 * np.sqrt((x1 - x0) * (y1 - y0) / n)
+        self.n, self.box, self.margin = n, (x0, x1, y0, y1), margin
+        p = _BoxPointsParams(n, None, None, x0, x1, y0, y1, margin)
+        self._lib = _lib()
+        self._h = self._lib.rm_mover_create(ctypes.byref(p))
+        if not self._h:
+            raise ValueError("rm_mover_create rejected the box")
+
+    def mesh(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if x.shape != (self.n,) or y.shape != (self.n,):
+            raise ValueError(f"expected {self.n} points")
+        h = self._lib.rm_mover_mesh(self._h, _ffi.ptr(x), _ffi.ptr(y))
+        if h:
+            return _export(h, RIGID, lattice_ij=False)
+        return _points_qhull(x, y, self.box, self.margin)
+
+    def counts(self):
+        """(full triangulations, flip repairs) so far."""
+        b, r = ctypes.c_int32(), ctypes.c_int32()
+        self._lib.rm_mover_counts(self._h, ctypes.byref(b), ctypes.byref(r))
+        return b.value, r.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rm_mover_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _points_qhull(x, y, box, margin):
+    from scipy.spatial import Delaunay
+    N = len(x)
+    x0, x1, y0, y1 = box
     gx, gy, go, gw = [], [], [], []
     for wall, near, fx, fy in (
             (0, x1 - x < margin, lambda: x + 2 * (x1 - x), lambda: y + 0.0),

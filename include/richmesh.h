@@ -60,7 +60,7 @@ void rm_free(rm_mesh* m);
  * caller supplies the real points [0, n_real), ghost points reflected across
  * a wall (ghost_orig: the real point reflected; ghost_wall: 0 right, 1 up,
  * 2 left, 3 down) and the Delaunay triangles of all of them (any
- * orientation).  Triangles are numbered in Hilbert order of their centroids;
+ * orientation).  Triangles are numbered by their smallest point id;
  * faces, their order and the cell geometry then follow the same rules as
  * rm_lattice_build.  Returns NULL on bad input. */
 typedef struct rm_points_params {
@@ -74,6 +74,45 @@ typedef struct rm_points_params {
 } rm_points_params;
 
 rm_mesh* rm_points_build(const rm_points_params* p);
+
+/* The mesh of points in a rigid-wall box in one call: ghosts are the
+ * reflections across each wall of the points within `margin` of it (walls
+ * in the order 0 right, 1 up, 2 left, 3 down, points in index order), the
+ * triangulation is rm_delaunay's, and the triangles with a real vertex and
+ * the ghosts they use make the mesh as in rm_points_build.  Returns NULL on
+ * bad input or when the points cannot be triangulated (e.g. duplicates). */
+typedef struct rm_box_points_params {
+  int32_t n;
+  const double *x, *y;            /* [n]                                 */
+  double x0, x1, y0, y1;
+  double margin;
+} rm_box_points_params;
+
+rm_mesh* rm_box_points_build(const rm_box_points_params* p);
+
+/* A moving mesh's rebuilds: rm_mover_mesh gives the mesh of the points'
+ * current positions (n, box and margin of rm_mover_create; its x, y are not
+ * read), the same mesh rm_box_points_build gives, up to the diagonal of
+ * cocircular point quads.  It keeps the previous triangulation of points and
+ * ghosts and, when the moved points leave every triangle counter-clockwise,
+ * the boundary convex and every point near a wall with its reflection,
+ * restores the Delaunay property by edge flips instead of triangulating
+ * afresh.  The caller frees each mesh with rm_free. */
+typedef struct rm_mover rm_mover;
+rm_mover* rm_mover_create(const rm_box_points_params* p);
+rm_mesh* rm_mover_mesh(rm_mover* mv, const double* x, const double* y);
+/* Full triangulations and flip repairs so far. */
+void rm_mover_counts(const rm_mover* mv, int32_t* builds, int32_t* repairs);
+void rm_mover_free(rm_mover* mv);
+
+/* Delaunay triangulation of n points (the host side of a moving-mesh
+ * rebuild; the reference inserts points one at a time,
+ * Delaunay::build_delaunay, Delaunay.cpp:296-402): radial sweep, then edge flips, exact orientation tests.
+ * Writes *n_tri <= tri_cap counter-clockwise triangles [3 n_tri] and returns
+ * 0, or a negative code (duplicate or unplaceable point, flip budget) for
+ * the caller to fall back on another triangulator. */
+int rm_delaunay(int32_t n, const double* x, const double* y, int32_t* tri,
+                int32_t tri_cap, int32_t* n_tri);
 
 /* Sizes: cells N, faces F, ghosts G, CSR entries nnz. */
 void rm_sizes(const rm_mesh* m, int32_t* n_cells, int32_t* n_faces,

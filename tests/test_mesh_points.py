@@ -1,8 +1,11 @@
-"""The point-set mesh builder (rich_amd.mesh.points: Qhull Delaunay of the
-points and their wall reflections, then rm_points_build) used for moving
-meshes: on lattice points it reproduces the lattice builder's mesh; on
+"""The point-set mesh builders used for moving meshes (rich_amd.mesh.points:
+rm_box_points_build, i.e. rm_delaunay of the points and their wall
+reflections, then the faces; BoxMesher: the previous triangulation repaired
+by flips): on lattice points they reproduce the lattice builder's mesh; on
 moved points the cells tile the box, faces are perpendicular bisectors and
-every wall face is the mirror face of its cell."""
+every wall face is the mirror face of its cell.  rm_delaunay is checked
+against Qhull and against the empty-circumcircle property by brute force."""
+import ctypes
 import numpy as np
 import pytest
 
@@ -63,3 +66,103 @@ def test_moved_points_mesh(shake):
         fs = m.csr_face[m.csr_off[i]:m.csr_off[i + 1]]
         assert np.all(np.diff(fs) > 0)
         assert np.all((a[fs] == i) | (b[fs] == i))
+
+
+def _tri_set(t):
+    return {tuple(sorted(r)) for r in np.asarray(t).tolist()}
+
+
+def _empty_circles(x, y, tri, rel=1e-9):
+    """Brute force: no point lies inside any triangle's circumcircle (beyond
+    round-off), and every triangle is counter-clockwise."""
+    a, b, c = (np.stack([x[tri[:, k]], y[tri[:, k]]], 1) for k in range(3))
+    cross = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    assert np.all(cross > 0)
+    B, C = b - a, c - a
+    d = 2 * (B[:, 0] * C[:, 1] - B[:, 1] * C[:, 0])
+    b2, c2 = (B * B).sum(1), (C * C).sum(1)
+    ux = (C[:, 1] * b2 - B[:, 1] * c2) / d
+    uy = (B[:, 0] * c2 - C[:, 0] * b2) / d
+    cx, cy, r2 = a[:, 0] + ux, a[:, 1] + uy, ux * ux + uy * uy
+    for i in range(len(x)):
+        d2 = (x[i] - cx) ** 2 + (y[i] - cy) ** 2
+        assert np.all(d2 >= r2 * (1 - rel)), i
+
+
+def test_delaunay_matches_qhull_in_general_position():
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(11)
+    for n in (3, 4, 10, 300, 5000):
+        x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        t = rmesh.delaunay(x, y)
+        assert _tri_set(t) == _tri_set(Delaunay(np.column_stack([x, y])).simplices)
+
+
+def test_delaunay_brute_force_and_degenerate_inputs():
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(0, 1, 400), rng.uniform(0, 1, 400)
+    _empty_circles(x, y, rmesh.delaunay(x, y))
+    # a square lattice: every quad cocircular, any diagonal is Delaunay
+    gx, gy = np.meshgrid(np.arange(12.0), np.arange(9.0))
+    x, y = gx.ravel(), gy.ravel()
+    t = rmesh.delaunay(x, y)
+    assert len(t) == 2 * 11 * 8
+    _empty_circles(x, y, t)
+    # points on a circle and collinear runs
+    a = np.linspace(0, 2 * np.pi, 40, endpoint=False)
+    x = np.concatenate([np.cos(a), np.linspace(-0.5, 0.5, 11), [0.0]])
+    y = np.concatenate([np.sin(a), np.zeros(11), [0.3]])
+    _empty_circles(x, y, rmesh.delaunay(x, y))
+
+
+def test_delaunay_rejects_duplicates():
+    lib = rmesh._lib()
+    x = np.array([0.0, 1.0, 0.0, 1.0, 1.0])
+    y = np.array([0.0, 0.0, 1.0, 1.0, 0.0])
+    tri = np.empty((10, 3), np.int32)
+    nt = ctypes.c_int32()
+    rc = lib.rm_delaunay(5, x.ctypes.data, y.ctypes.data, tri.ctypes.data, 10, ctypes.byref(nt))
+    assert rc < 0
+    # the Python entry falls back on Qhull, which drops the duplicate
+    assert len(rmesh.delaunay(x, y)) == 2
+
+
+def _same_mesh(a, b):
+    pa = {tuple(sorted(p)) for p in zip(a.face_first, a.face_second)}
+    pb = {tuple(sorted(p)) for p in zip(b.face_first, b.face_second)}
+    assert (a.n_faces, a.n_ghosts) == (b.n_faces, b.n_ghosts)
+    assert pa == pb
+    np.testing.assert_allclose(a.volume, b.volume, rtol=1e-12)
+    np.testing.assert_allclose(a.tess_volume, b.tess_volume, rtol=1e-12)
+    np.testing.assert_allclose(a.cm_x, b.cm_x, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(a.cm_y, b.cm_y, rtol=0, atol=1e-13)
+
+
+def test_points_equals_qhull_path():
+    m0 = rmesh.lattice(30, 30, BOX, jitter=1e-3, seed=2)
+    N = m0.n_cells
+    rng = np.random.default_rng(8)
+    x = np.clip(m0.pt_x[:N] + 0.2 * (2 / 30) * rng.uniform(-1, 1, N), -0.999, 0.999)
+    y = np.clip(m0.pt_y[:N] + 0.2 * (2 / 30) * rng.uniform(-1, 1, N), -0.999, 0.999)
+    a = rmesh.points(x, y, BOX)
+    b = rmesh._points_qhull(x, y, BOX, 3.0 * np.sqrt(4.0 / N))
+    _same_mesh(a, b)
+
+
+def test_box_mesher_repairs_match_fresh_builds():
+    """Small steps are repaired by flips, a large jump triggers a fresh
+    triangulation; every mesh equals the fresh one of the same points."""
+    m0 = rmesh.lattice(40, 40, BOX, jitter=1e-3, seed=6)
+    N = m0.n_cells
+    x, y = m0.pt_x[:N].copy(), m0.pt_y[:N].copy()
+    B = rmesh.BoxMesher(N, BOX)
+    rng = np.random.default_rng(9)
+    h = 2.0 / 40
+    for step in range(12):
+        _same_mesh(B.mesh(x, y), rmesh.points(x, y, BOX))
+        amp = 0.6 if step == 7 else 0.03  # step 8 scrambles the points
+        x = np.clip(x + amp * h * rng.standard_normal(N), -0.999, 0.999)
+        y = np.clip(y + amp * h * rng.standard_normal(N), -0.999, 0.999)
+    builds, repairs = B.counts()
+    assert builds >= 2 and repairs >= 5, (builds, repairs)
+    B.close()
