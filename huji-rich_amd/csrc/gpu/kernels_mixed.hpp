@@ -393,17 +393,17 @@ __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArg
     const D4 x = a.in[i];
     KSum sm{x.a, 0}, sE{x.b, 0}, sx{x.c, 0}, sy{x.d, 0};
     const int deg = m.off[i + 1] - m.off[i], eb = ell_base(t, i);
-    for (int s0 = 0; s0 < deg; s0 += CH) {
-      unsigned inf[CH];
-      F4 fl[CH];
+    for (int s0 = 0; s0 < deg; s0 += CH_UPD) {
+      unsigned inf[CH_UPD];
+      F4 fl[CH_UPD];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
+      for (int u = 0; u < CH_UPD; ++u)
         if (s0 + u < deg) inf[u] = t.einfo[eb + 64 * (s0 + u)];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
+      for (int u = 0; u < CH_UPD; ++u)
         if (s0 + u < deg) fl[u] = a.flux[inf[u] & INFO_FACE];
 #pragma unroll
-      for (int u = 0; u < CH; ++u) {
+      for (int u = 0; u < CH_UPD; ++u) {
         if (s0 + u >= deg) break;
         const double s = (inf[u] & INFO_SECOND) ? dt : -dt;
         sm.add(s * static_cast<double>(fl[u].a));

@@ -83,7 +83,10 @@ struct rg_ctx {
   int dm_ne = 0;  // ELL entries
   bool has_mesh = false, has_cells = false, has_ext = false, has_flux = false;
 
-  std::vector<void*> mesh_bufs;  // freed on re-mesh
+  std::vector<std::pair<void*, size_t>> mesh_bufs;  // released on re-mesh
+  // buffers of the previous mesh while a new one is set, reused where the
+  // size fits (a moving mesh's tables keep nearly the same sizes)
+  std::vector<std::pair<void*, size_t>> pool;
   int64_t bytes = 0;
 
   DMesh dm{};
@@ -158,13 +161,33 @@ int fail_hip(rg_ctx* c, hipError_t e) {
 template <class T>
 hipError_t dalloc(rg_ctx* c, T** p, size_t n) {
   void* q = nullptr;
-  const size_t bytes = sizeof(T) * (n ? n : 1);
-  const hipError_t e = hipMalloc(&q, bytes);
-  if (e != hipSuccess) return e;
-  c->mesh_bufs.push_back(q);
+  size_t bytes = sizeof(T) * (n ? n : 1);
+  // the smallest pooled buffer that holds it without wasting over a quarter
+  size_t best = c->pool.size();
+  for (size_t k = 0; k < c->pool.size(); ++k) {
+    const size_t b = c->pool[k].second;
+    if (b >= bytes && b <= bytes + bytes / 4 + 4096 &&
+        (best == c->pool.size() || b < c->pool[best].second))
+      best = k;
+  }
+  if (best < c->pool.size()) {
+    q = c->pool[best].first;
+    bytes = c->pool[best].second;
+    c->pool[best] = c->pool.back();
+    c->pool.pop_back();
+  } else {
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return e;
+  }
+  c->mesh_bufs.emplace_back(q, bytes);
   c->bytes += static_cast<int64_t>(bytes);
   *p = static_cast<T*>(q);
   return hipSuccess;
+}
+
+void drain_pool(rg_ctx* c) {
+  for (const auto& b : c->pool) (void)hipFree(b.first);
+  c->pool.clear();
 }
 
 template <class T>
@@ -175,8 +198,15 @@ hipError_t upload(rg_ctx* c, T** p, const T* src, size_t n) {
   return e;
 }
 
-void free_mesh(rg_ctx* c) {
-  for (void* p : c->mesh_bufs) (void)hipFree(p);
+// Release the mesh's buffers: freed, or with `pool` kept for the next
+// mesh's dalloc (the caller has synchronized the stream).
+void free_mesh(rg_ctx* c, bool pool = false) {
+  for (const auto& b : c->mesh_bufs) {
+    if (pool)
+      c->pool.push_back(b);
+    else
+      (void)hipFree(b.first);
+  }
   c->mesh_bufs.clear();
   c->bytes = 0;
   c->prim = c->mid_prim = c->ext = c->mid_ext = c->flux = c->left = c->right = nullptr;
@@ -591,6 +621,7 @@ void rg_destroy(rg_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_mesh(c);
+  drain_pool(c);
   if (c->pv) (void)hipFree(c->pv);
   if (c->dt_hist) (void)hipFree(c->dt_hist);
   if (c->words) (void)hipFree(c->words);
@@ -690,7 +721,8 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   for (int i = 0; i < N + G; ++i) cm[i] = D2{m->cm_x[i], m->cm_y[i]};
   // --- upload
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  free_mesh(c);
+  drain_pool(c);
+  free_mesh(c, true);
   const size_t NG = static_cast<size_t>(N) + G;
   DMesh d{};
   d.N = N;
@@ -940,6 +972,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   c->G = G;
   c->nnz = nnz;
   c->has_mesh = true;
+  drain_pool(c);
   return RG_OK;
 }
 
