@@ -10,7 +10,7 @@ from rich_amd import _ffi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
-HEADERS = {"richgpu.h": "librichgpu", "richmesh.h": "librichmesh"}
+HEADERS = {"richgpu.h": "librichgpu", "richmesh.h": "librichmesh", "richio.h": "librichio"}
 
 
 def declared_functions(header):
@@ -18,13 +18,15 @@ def declared_functions(header):
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
     names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\([^;{]*\)\s*;", text)
-    return sorted(set(n for n in names if n.startswith(("rg_", "rm_"))))
+    return sorted(set(n for n in names if n.startswith(("rg_", "rm_", "rio_"))))
 
 
 @pytest.mark.parametrize("header", sorted(HEADERS))
 def test_exports(header):
     names = declared_functions(header)
     assert len(names) >= 5
+    if header == "richio.h" and not os.path.exists(os.path.join(_ffi.LIB_DIR, "librichio.so")):
+        pytest.skip("built without HDF5")
     lib = _ffi.load(HEADERS[header])
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
