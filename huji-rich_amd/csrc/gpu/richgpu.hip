@@ -127,6 +127,17 @@ struct rg_ctx {
   double *ens_inv = nullptr, *ens_hist = nullptr;
   hipGraphExec_t ens_graph = nullptr;
 
+  // what the adjoint tables are built from, kept on the host until the
+  // first rg_adjoint_forward of the mesh needs them
+  struct AdjSource {
+    std::vector<Inc> inc;        // [nnz]
+    std::vector<int> csr_off;    // [N+1]
+    std::vector<int> ff, fs;     // [F]
+    std::vector<int> gsrc;       // [G]
+    std::vector<int> wofs;       // [waves+1]
+    std::vector<int> epos;       // [2F] ELL entry of each face's first and second side
+  } adj_src;
+
   // moving mesh: point velocities kept across rg_replace_mesh
   D2* pv = nullptr;
   int pv_cap = 0;
@@ -217,6 +228,7 @@ void free_mesh(rg_ctx* c, bool pool = false) {
   for (void* p : c->adj_bufs) (void)hipFree(p);
   c->adj_bufs.clear();
   c->am = AdjMesh{};
+  c->adj_src = rg_ctx::AdjSource{};
   c->adj_steps = c->adj_every = c->adj_nck = c->adj_cap_every = c->adj_cap_nck = 0;
   c->ckP = c->ckX = c->finP = c->finX = c->tP = c->tX = c->tPm = c->tXm = nullptr;
   c->tR1 = c->tR2 = nullptr;
@@ -529,6 +541,64 @@ hipError_t ens_capture(rg_ctx* c, int block, int bpc) {
   return e;
 }
 
+// Adjoint tables of the current mesh.  rev: for each incidence (i, s) with a
+// real neighbour j, the ELL entry of the same face in j's list.  gf: the
+// faces whose ghost side copies cell b (b = the ghost's source), with the
+// ghost's side.  ge: the incidences whose neighbour is a ghost copying b.
+hipError_t build_adjoint_tables(rg_ctx* c) {
+  const rg_ctx::AdjSource& a = c->adj_src;
+  const int N = c->N, F = c->F;
+  const size_t ne = static_cast<size_t>(c->dm_ne);
+  auto ell_at = [&](int i, int k) {
+    return static_cast<size_t>(a.wofs[i / 64] + i % 64) +
+           64 * static_cast<size_t>(k - a.csr_off[i]);
+  };
+  std::vector<int> rev(ne, -1);
+  std::vector<int> gf_off(static_cast<size_t>(N) + 1, 0), ge_off(static_cast<size_t>(N) + 1, 0);
+  for (int f = 0; f < F; ++f) {
+    if (a.ff[f] >= N) ++gf_off[a.gsrc[a.ff[f] - N] + 1];
+    if (a.fs[f] >= N) ++gf_off[a.gsrc[a.fs[f] - N] + 1];
+  }
+  for (int i = 0; i < N; ++i)
+    for (int k = a.csr_off[i]; k < a.csr_off[i + 1]; ++k) {
+      const int j = a.inc[k].nbr;
+      if (j >= N) {
+        ++ge_off[a.gsrc[j - N] + 1];
+        continue;
+      }
+      const unsigned w = a.inc[k].info;
+      rev[a.epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 0 : 1)]] =
+          static_cast<int>(ell_at(i, k));
+    }
+  for (int i = 0; i < N; ++i) {
+    gf_off[i + 1] += gf_off[i];
+    ge_off[i + 1] += ge_off[i];
+  }
+  std::vector<unsigned> gf(static_cast<size_t>(std::max(gf_off[N], 1)), 0u);
+  std::vector<int> ge(static_cast<size_t>(std::max(ge_off[N], 1)), 0);
+  {
+    std::vector<int> fill(gf_off.begin(), gf_off.end() - 1);
+    for (int f = 0; f < F; ++f) {
+      if (a.ff[f] >= N) gf[fill[a.gsrc[a.ff[f] - N]]++] = static_cast<unsigned>(f);
+      if (a.fs[f] >= N) gf[fill[a.gsrc[a.fs[f] - N]]++] = static_cast<unsigned>(f) | INFO_SECOND;
+    }
+    std::vector<int> gfill(ge_off.begin(), ge_off.end() - 1);
+    for (int i = 0; i < N; ++i)
+      for (int k = a.csr_off[i]; k < a.csr_off[i + 1]; ++k)
+        if (a.inc[k].nbr >= N) ge[gfill[a.gsrc[a.inc[k].nbr - N]]++] = static_cast<int>(ell_at(i, k));
+  }
+  int *rev_d, *gf_off_d, *ge_off_d, *ge_d;
+  unsigned* gf_d;
+  hipError_t e = upload(c, &rev_d, rev.data(), ne);
+  if (e == hipSuccess) e = upload(c, &gf_off_d, gf_off.data(), gf_off.size());
+  if (e == hipSuccess) e = upload(c, &gf_d, gf.data(), gf.size());
+  if (e == hipSuccess) e = upload(c, &ge_off_d, ge_off.data(), ge_off.size());
+  if (e == hipSuccess) e = upload(c, &ge_d, ge.data(), ge.size());
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host vectors above
+  if (e == hipSuccess) c->am = AdjMesh{rev_d, gf_off_d, gf_d, ge_off_d, ge_d};
+  return e;
+}
+
 // hdsim::TimeAdvance2Heun with FP32 reconstruction and HLLC and FP64
 // extensives (RG_MIXED).  The FP64 cells are recovered at the end.
 int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
@@ -776,56 +846,19 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       enbr[ell_at(i, k)] = inc[k].nbr;
       einfo[ell_at(i, k)] = inc[k].info;
     }
-  // Adjoint tables.  rev: for each incidence (i, s) with a real neighbour j,
-  // the ELL entry of the same face in j's list.  gf: the faces whose ghost
-  // side copies cell b (b = the ghost's source), with the ghost's side.
-  // ge: the incidences whose neighbour is a ghost copying cell b.
-  std::vector<int> rev(ne, -1);
-  std::vector<int> gf_off(static_cast<size_t>(N) + 1, 0), ge_off(static_cast<size_t>(N) + 1, 0);
-  for (int f = 0; f < F; ++f) {
-    const int a = m->face_first[f], b = m->face_second[f];
-    if (a >= N) ++gf_off[gsrc[a - N] + 1];
-    if (b >= N) ++gf_off[gsrc[b - N] + 1];
-  }
+  // ELL entry of each face on its first [2f] and second [2f + 1] side; every
+  // face between two real cells must be in both cells' lists
+  std::vector<int> epos(2 * static_cast<size_t>(F), -1);
   for (int i = 0; i < N; ++i)
     for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-      const int j = inc[k].nbr;
-      if (j >= N) {
-        ++ge_off[gsrc[j - N] + 1];
-        continue;
-      }
-      const int f = static_cast<int>(inc[k].info & INFO_FACE);
-      int kj = m->csr_off[j];
-      while (kj < m->csr_off[j + 1] && m->csr_face[kj] != f) ++kj;
-      if (kj == m->csr_off[j + 1]) return RG_E_ARG;  // face missing from the neighbour's list
-      rev[ell_at(j, kj)] = static_cast<int>(ell_at(i, k));
+      const unsigned w = inc[k].info;
+      epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 1 : 0)] =
+          static_cast<int>(ell_at(i, k));
     }
-  for (int i = 0; i < N; ++i) {
-    gf_off[i + 1] += gf_off[i];
-    ge_off[i + 1] += ge_off[i];
-  }
-  std::vector<unsigned> gf(static_cast<size_t>(std::max(gf_off[N], 1)), 0u);
-  std::vector<int> ge(static_cast<size_t>(std::max(ge_off[N], 1)), 0);
-  {
-    std::vector<int> fill(gf_off.begin(), gf_off.end() - 1);
-    for (int f = 0; f < F; ++f) {
-      const int a = m->face_first[f], b = m->face_second[f];
-      if (a >= N) gf[fill[gsrc[a - N]]++] = static_cast<unsigned>(f);
-      if (b >= N) gf[fill[gsrc[b - N]]++] = static_cast<unsigned>(f) | INFO_SECOND;
-    }
-    std::vector<int> gfill(ge_off.begin(), ge_off.end() - 1);
-    for (int i = 0; i < N; ++i)
-      for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k)
-        if (inc[k].nbr >= N) ge[gfill[gsrc[inc[k].nbr - N]]++] = static_cast<int>(ell_at(i, k));
-  }
-  int *rev_d, *gf_off_d, *ge_off_d, *ge_d;
-  unsigned* gf_d;
-  HIPCHK(c, upload(c, &rev_d, rev.data(), ne));
-  HIPCHK(c, upload(c, &gf_off_d, gf_off.data(), gf_off.size()));
-  HIPCHK(c, upload(c, &gf_d, gf.data(), gf.size()));
-  HIPCHK(c, upload(c, &ge_off_d, ge_off.data(), ge_off.size()));
-  HIPCHK(c, upload(c, &ge_d, ge.data(), ge.size()));
-  c->am = AdjMesh{rev_d, gf_off_d, gf_d, ge_off_d, ge_d};
+  for (int k = 0; k < nnz; ++k)
+    if (inc[k].nbr < N &&
+        epos[2 * static_cast<size_t>(inc[k].info & INFO_FACE) + ((inc[k].info & INFO_SECOND) ? 0 : 1)] < 0)
+      return RG_E_ARG;  // face missing from the neighbour's list
   int *wofs_d, *enbr_d;
   unsigned* einfo_d;
   D2 *eh, *eo;
@@ -971,6 +1004,13 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   c->F = F;
   c->G = G;
   c->nnz = nnz;
+  c->adj_src.inc = std::move(inc);
+  c->adj_src.csr_off.assign(m->csr_off, m->csr_off + N + 1);
+  c->adj_src.ff.assign(m->face_first, m->face_first + F);
+  c->adj_src.fs.assign(m->face_second, m->face_second + F);
+  c->adj_src.gsrc.assign(gsrc.begin(), gsrc.begin() + G);
+  c->adj_src.wofs = std::move(wofs);
+  c->adj_src.epos = std::move(epos);
   c->has_mesh = true;
   drain_pool(c);
   return RG_OK;
@@ -1206,6 +1246,7 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
   if (!ready(c, true, true) || c->prm.order != 2 || c->prm.precision != RG_FP64 || c->dm.wx ||
       nsteps <= 0 || nsteps >= (1 << 24) || every <= 0 || !c->ext_init)
     return RG_E_ARG;
+  if (!c->am.rev) HIPCHK(c, build_adjoint_tables(c));
   const int N = c->N, F = c->F;
   const int nck = (nsteps + every - 1) / every, ev = std::min(every, nsteps);
   // buffers sized per run (reallocated when a run needs more)
