@@ -1461,7 +1461,23 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
                          int32_t* rows_out) {
   if (!ready(c, true, true) || c->ens_K <= 0 || trial_steps <= 0) return RG_E_ARG;
   const size_t bytes = sizeof(D4) * static_cast<size_t>(c->N);
-  D4 *saveP = nullptr, *saveX = nullptr;
+  // released on every return: the saved state, the timing events, and the
+  // profiling switch (off while the shapes are timed)
+  struct Scope {
+    rg_ctx* c;
+    bool prof;
+    D4 *saveP = nullptr, *saveX = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    ~Scope() {
+      if (saveP) (void)hipFree(saveP);
+      if (saveX) (void)hipFree(saveX);
+      if (t0) (void)hipEventDestroy(t0);
+      if (t1) (void)hipEventDestroy(t1);
+      c->prof = prof;
+    }
+  } sc{c, c->prof};
+  D4*& saveP = sc.saveP;
+  D4*& saveX = sc.saveX;
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&saveP), bytes));
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&saveX), bytes));
   HIPCHK(c, hipMemcpyAsync(saveP, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
@@ -1474,9 +1490,9 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
                         sizeof(double) * static_cast<size_t>(trial_steps + 1) * c->ens_K));
     c->ens_hist_cap = trial_steps + 1;
   }
-  const bool prof = c->prof;
   c->prof = false;
-  hipEvent_t t0, t1;
+  hipEvent_t& t0 = sc.t0;
+  hipEvent_t& t1 = sc.t1;
   HIPCHK(c, hipEventCreate(&t0));
   HIPCHK(c, hipEventCreate(&t1));
   const int blocks[3] = {64, 128, 256};
@@ -1514,14 +1530,10 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
         best_bpc = bpc;
       }
     }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   HIPCHK(c, hipMemcpyAsync(c->prim, saveP, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->ext, saveX, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  (void)hipFree(saveP);
-  (void)hipFree(saveX);
-  c->prof = prof;
+  c->prof = sc.prof;  // the kept graph is captured as the caller profiles
   HIPCHK(c, ens_capture(c, best_block, best_bpc));
   if (rows_out) *rows_out = rows;
   return collect_error(c);
