@@ -19,6 +19,8 @@
 // use another triangulator.
 #include "delaunay.hpp"
 
+#include "parallel.hpp"
+
 #include "../../../include/richmesh.h"
 
 #include <algorithm>
@@ -142,9 +144,18 @@ struct Sweep {
   std::vector<int32_t> stack;
   int64_t flips = 0, budget = 0;
 
-  // Flip half-edge e (and its twin) if the opposite point of the twin lies
-  // in the circumcircle; pushes the edges to check next: the two facing the
-  // apex of e's triangle (insertion) or all four outer edges.
+  // The twin's opposite point lies in the circumcircle of e's triangle.
+  bool illegal(int e) const {
+    const int f = half[e];
+    if (f < 0) return false;
+    const int t = e - e % 3, k = e % 3;
+    const int a = tri[e], b = tri[t + (k + 1) % 3], c = tri[t + (k + 2) % 3];
+    const int d = tri[f - f % 3 + (f % 3 + 2) % 3];
+    return in_circle(x[a], y[a], x[b], y[b], x[c], y[c], x[d], y[d]) != 0;
+  }
+  // Flip half-edge e (and its twin) if it is illegal; pushes the edges to
+  // check next: the two facing the apex of e's triangle (insertion) or all
+  // four outer edges.
   int flip(int e, bool all);
   int drain(bool all) {
     while (!stack.empty()) {
@@ -158,13 +169,12 @@ struct Sweep {
 };
 
 int Sweep::flip(int e, bool all) {
+  if (!illegal(e)) return 0;
   const int f = half[e];
-  if (f < 0) return 0;
   const int t = e - e % 3, k = e % 3;
   const int u = f - f % 3, j = f % 3;
   const int a = tri[e], b = tri[t + (k + 1) % 3], c = tri[t + (k + 2) % 3];
   const int d = tri[u + (j + 2) % 3];
-  if (!in_circle(x[a], y[a], x[b], y[b], x[c], y[c], x[d], y[d])) return 0;
   if (++flips > budget) return -4;
   const int Tbc = half[t + (k + 1) % 3], Tca = half[t + (k + 2) % 3];
   const int Uad = half[u + (j + 1) % 3], Udb = half[u + (j + 2) % 3];
@@ -373,9 +383,20 @@ int rmd::repair(int32_t n, const double* x, const double* y, std::vector<int32_t
   const int32_t T = static_cast<int32_t>(tri.size() / 3);
   if (n < 3 || !x || !y || half.size() != tri.size() || T < 1) return -1;
   // every triangle still counter-clockwise
-  for (int32_t t = 0; t < T; ++t) {
-    const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
-    if (orient(x[a], y[a], x[b], y[b], x[c], y[c]) <= 0) return -6;
+  {
+    const int nc = rmp::chunks(T);
+    std::vector<char> bad(static_cast<size_t>(nc), 0);
+    rmp::for_chunks(T, nc, [&](int k, int64_t b0, int64_t e0) {
+      for (int64_t t = b0; t < e0; ++t) {
+        const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+        if (orient(x[a], y[a], x[b], y[b], x[c], y[c]) <= 0) {
+          bad[k] = 1;
+          return;
+        }
+      }
+    });
+    for (char v : bad)
+      if (v) return -6;
   }
   // the boundary (edges with no twin, interior on their left) still convex
   std::vector<int32_t> hn(static_cast<size_t>(n), -1);
@@ -397,8 +418,17 @@ int rmd::repair(int32_t n, const double* x, const double* y, std::vector<int32_t
   s.budget = 64 * static_cast<int64_t>(n) + 1024;
   s.tri.swap(tri);
   s.half.swap(half);
-  for (int32_t e = 0; e < 3 * T; ++e)
-    if (s.half[e] > e) s.stack.push_back(e);
+  // the edges no longer locally Delaunay (found in parallel, kept in edge
+  // order), then flips from them
+  {
+    const int nc = rmp::chunks(3 * static_cast<int64_t>(T));
+    std::vector<std::vector<int32_t>> bad(static_cast<size_t>(nc));
+    rmp::for_chunks(3 * static_cast<int64_t>(T), nc, [&](int k, int64_t b0, int64_t e0) {
+      for (int64_t e = b0; e < e0; ++e)
+        if (s.half[e] > e && s.illegal(static_cast<int>(e))) bad[k].push_back(static_cast<int32_t>(e));
+    });
+    for (int k = 0; k < nc; ++k) s.stack.insert(s.stack.end(), bad[k].begin(), bad[k].end());
+  }
   const int rc = s.drain(true);
   tri.swap(s.tri);
   half.swap(s.half);

@@ -9,6 +9,7 @@
 // side, which is all a real cell's Voronoi polygon depends on.
 #include "../../../include/richmesh.h"
 #include "delaunay.hpp"
+#include "parallel.hpp"
 
 #include <cmath>
 #include <cstdint>
@@ -111,27 +112,66 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
     std::sort(edges.begin(), edges.end(), before);
   // 6. faces, dropping near-zero-length ones (VoronoiMesh.cpp:287-288)
   uint64_t fs = static_cast<uint64_t>(flip_seed) * 0x9E3779B97F4A7C15ull + 12345u;
-  m->ff.reserve(edges.size());
-  m->fs.reserve(edges.size());
-  m->v1x.reserve(edges.size());
-  m->v1y.reserve(edges.size());
-  m->v2x.reserve(edges.size());
-  m->v2y.reserve(edges.size());
-  for (const EdgeRef& e : edges) {
+  auto long_enough = [&](const EdgeRef& e) {
     const Tri &tl = tris[e.tlo], &th = tris[e.thi];
     const double lx = th.cc.x - tl.cc.x, ly = th.cc.y - tl.cc.y;
     const double len = std::sqrt(lx * lx + ly * ly);
-    if (!(len > 1e-8 * std::sqrt(tl.r * th.r))) continue;
-    int32_t first = e.first, second = e.second;
-    if (flip_seed != 0 && (splitmix64(fs) & 1u)) std::swap(first, second);
-    const int32_t f = static_cast<int32_t>(m->ff.size());
-    m->ff.push_back(first);
-    m->fs.push_back(second);
-    m->v1x.push_back(tl.cc.x);
-    m->v1y.push_back(tl.cc.y);
-    m->v2x.push_back(th.cc.x);
-    m->v2y.push_back(th.cc.y);
-    (void)f;
+    return len > 1e-8 * std::sqrt(tl.r * th.r);
+  };
+  if (flip_seed == 0) {
+    // kept faces counted per chunk, then written in place
+    const int64_t ne = static_cast<int64_t>(edges.size());
+    const int nc = rmp::chunks(ne);
+    std::vector<int64_t> cnt(static_cast<size_t>(nc) + 1, 0);
+    rmp::for_chunks(ne, nc, [&](int c, int64_t e0, int64_t e1) {
+      int64_t k = 0;
+      for (int64_t i = e0; i < e1; ++i) k += long_enough(edges[i]) ? 1 : 0;
+      cnt[c + 1] = k;
+    });
+    for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+    const size_t F = static_cast<size_t>(cnt[nc]);
+    m->ff.resize(F);
+    m->fs.resize(F);
+    m->v1x.resize(F);
+    m->v1y.resize(F);
+    m->v2x.resize(F);
+    m->v2y.resize(F);
+    rmp::for_chunks(ne, nc, [&](int c, int64_t e0, int64_t e1) {
+      int64_t f = cnt[c];
+      for (int64_t i = e0; i < e1; ++i) {
+        const EdgeRef& e = edges[i];
+        if (!long_enough(e)) continue;
+        const Tri &tl = tris[e.tlo], &th = tris[e.thi];
+        m->ff[f] = e.first;
+        m->fs[f] = e.second;
+        m->v1x[f] = tl.cc.x;
+        m->v1y[f] = tl.cc.y;
+        m->v2x[f] = th.cc.x;
+        m->v2y[f] = th.cc.y;
+        ++f;
+      }
+    });
+  }
+  else {
+    // a seeded face orientation draws one number per kept face, in order
+    m->ff.reserve(edges.size());
+    m->fs.reserve(edges.size());
+    m->v1x.reserve(edges.size());
+    m->v1y.reserve(edges.size());
+    m->v2x.reserve(edges.size());
+    m->v2y.reserve(edges.size());
+    for (const EdgeRef& e : edges) {
+      if (!long_enough(e)) continue;
+      const Tri &tl = tris[e.tlo], &th = tris[e.thi];
+      int32_t first = e.first, second = e.second;
+      if (splitmix64(fs) & 1u) std::swap(first, second);
+      m->ff.push_back(first);
+      m->fs.push_back(second);
+      m->v1x.push_back(tl.cc.x);
+      m->v1y.push_back(tl.cc.y);
+      m->v2x.push_back(th.cc.x);
+      m->v2y.push_back(th.cc.y);
+    }
   }
   m->F = static_cast<int32_t>(m->ff.size());
   // CSR: each cell's faces in ascending face order
@@ -155,7 +195,8 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
   m->tvol.resize(N);
   m->cmx.resize(static_cast<size_t>(N) + G);
   m->cmy.resize(static_cast<size_t>(N) + G);
-  for (int32_t c = 0; c < N; ++c) {
+  rmp::parallel_for(N, [&](int64_t cc) {
+    const int32_t c = static_cast<int32_t>(cc);
     const int32_t b = m->off[c], e = m->off[c + 1];
     // Tessellation::GetVolume: fan about the generating point
     double area = 0;
@@ -187,7 +228,7 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
     m->vol[c] = vol;
     m->cmx[c] = pcx / a2;
     m->cmy[c] = pcy / a2;
-  }
+  });
   for (int32_t g = 0; g < G; ++g) {
     const int32_t o = m->gorig[g];
     const int32_t id = N + g;
@@ -498,21 +539,25 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
       const int k = q[0] <= q[1] && q[0] <= q[2] ? 0 : (q[1] <= q[2] ? 1 : 2);
       return std::make_pair(q[(k + 1) % 3], q[(k + 2) % 3]);
     };
-    size_t b = 0;
-    while (b < order.size()) {
-      size_t e = b + 1;
-      while (e < order.size() && order[e].first == order[b].first) ++e;
-      std::sort(order.begin() + b, order.begin() + e,
-                [&](const std::pair<int32_t, int32_t>& u, const std::pair<int32_t, int32_t>& v) {
-                  return next2(u.second) < next2(v.second);
-                });
-      b = e;
-    }
+    // each chunk sorts the buckets that start inside it
+    const int64_t no = static_cast<int64_t>(order.size());
+    rmp::for_chunks(no, rmp::chunks(no), [&](int, int64_t c0, int64_t c1) {
+      int64_t b = c0;
+      while (b > 0 && b < no && order[b].first == order[b - 1].first) ++b;
+      while (b < c1) {
+        int64_t e = b + 1;
+        while (e < no && order[e].first == order[b].first) ++e;
+        std::sort(order.begin() + b, order.begin() + e,
+                  [&](const std::pair<int32_t, int32_t>& u, const std::pair<int32_t, int32_t>& v) {
+                    return next2(u.second) < next2(v.second);
+                  });
+        b = e;
+      }
+    });
   }
-  std::vector<Tri> tris;
-  tris.reserve(order.size());
+  std::vector<Tri> tris(order.size());
   std::vector<int8_t> rot(half ? order.size() : 0);
-  for (size_t r = 0; r < order.size(); ++r) {
+  rmp::parallel_for(static_cast<int64_t>(order.size()), [&](int64_t r) {
     const int32_t* q = tri + 3 * static_cast<size_t>(order[r].second);
     int32_t a = q[0], b = q[1], c = q[2];
     if (!half) {
@@ -547,27 +592,35 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
     const double uy = (B.x * c2 - C.x * b2) / d;
     t.cc = V2{p0.x + ux, p0.y + uy};
     t.r = std::sqrt(ux * ux + uy * uy);
-    tris.push_back(t);
-  }
+    tris[r] = t;
+  });
   // shared edges: the first triangle (in the order above) to list an edge
   // owns it; the second one closes it
   std::vector<EdgeRef> edges;
   edges.reserve(tris.size() * 3 / 2);
   if (half) {
     std::vector<int32_t> rank(static_cast<size_t>(n_tri));
-    for (int32_t r = 0; r < n_tri; ++r) rank[order[r].second] = r;
-    for (int32_t r = 0; r < n_tri; ++r) {
-      const int32_t t0 = order[r].second;
-      for (int k = 0; k < 3; ++k) {
-        const int32_t h = half[3 * static_cast<size_t>(t0) + (k + rot[r]) % 3];
-        if (h < 0) continue;
-        const int32_t r2 = rank[h / 3];
-        if (r2 <= r) continue;
-        const int32_t a = tris[r].p[k], b = tris[r].p[(k + 1) % 3];
-        if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
-        edges.push_back(EdgeRef{r, r2, a, b, k});
+    rmp::parallel_for(n_tri, [&](int64_t r) { rank[order[r].second] = static_cast<int32_t>(r); });
+    // per chunk of triangles, joined in chunk order (the sequential order)
+    const int nc = rmp::chunks(n_tri);
+    std::vector<std::vector<EdgeRef>> part(static_cast<size_t>(nc));
+    rmp::for_chunks(n_tri, nc, [&](int c, int64_t r0, int64_t r1) {
+      part[c].reserve(static_cast<size_t>((r1 - r0) * 3 / 2 + 16));
+      for (int64_t rr = r0; rr < r1; ++rr) {
+        const int32_t r = static_cast<int32_t>(rr);
+        const int32_t t0 = order[r].second;
+        for (int k = 0; k < 3; ++k) {
+          const int32_t h = half[3 * static_cast<size_t>(t0) + (k + rot[r]) % 3];
+          if (h < 0) continue;
+          const int32_t r2 = rank[h / 3];
+          if (r2 <= r) continue;
+          const int32_t a = tris[r].p[k], b = tris[r].p[(k + 1) % 3];
+          if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
+          part[c].push_back(EdgeRef{r, r2, a, b, k});
+        }
       }
-    }
+    });
+    for (auto& v : part) edges.insert(edges.end(), v.begin(), v.end());
   } else {
     std::unordered_map<uint64_t, std::pair<int32_t, int32_t>> open;  // key -> (triangle, slot)
     open.reserve(tris.size() * 2);
@@ -671,8 +724,8 @@ rm_mesh* mesh_of_triangulation(int32_t N, const std::vector<double>& X,
       gwall.push_back(gw[g]);
     }
   std::vector<int32_t> t2(3 * static_cast<size_t>(nt)), h2(3 * static_cast<size_t>(nt));
-  for (int32_t t = 0; t < T; ++t) {
-    if (keep[t] < 0) continue;
+  rmp::parallel_for(T, [&](int64_t t) {
+    if (keep[t] < 0) return;
     for (int k = 0; k < 3; ++k) {
       const size_t e = 3 * static_cast<size_t>(t) + k, e2 = 3 * static_cast<size_t>(keep[t]) + k;
       const int32_t v = tri[e];
@@ -680,7 +733,7 @@ rm_mesh* mesh_of_triangulation(int32_t N, const std::vector<double>& X,
       const int32_t h = half[e];
       h2[e2] = (h >= 0 && keep[h / 3] >= 0) ? 3 * keep[h / 3] + h % 3 : -1;
     }
-  }
+  });
   return points_mesh(N, G, px.data(), py.data(), gorig.data(), gwall.data(), nt, t2.data(),
                      h2.data(), x0, x1, y0, y1);
 }
