@@ -166,3 +166,42 @@ def test_box_mesher_repairs_match_fresh_builds():
     builds, repairs = B.counts()
     assert builds >= 2 and repairs >= 5, (builds, repairs)
     B.close()
+
+
+_SEQ = """
+import sys, numpy as np
+sys.path.insert(0, {pkg!r})
+from rich_amd import mesh as rmesh
+m0 = rmesh.lattice(300, 300, (-1.0, 1.0, -1.0, 1.0), jitter=1e-3, seed=12)
+N = m0.n_cells
+x, y = m0.pt_x[:N].copy(), m0.pt_y[:N].copy()
+B = rmesh.BoxMesher(N, (-1.0, 1.0, -1.0, 1.0))
+rng = np.random.default_rng(1)
+out = {{}}
+for s in range(3):
+    m = B.mesh(x, y)
+    for k in ("face_first", "face_second", "v1x", "v2y", "csr_face", "volume", "cm_x"):
+        out[f"{{k}}_{{s}}"] = getattr(m, k)
+    x = np.clip(x + 0.05 * (2 / 300) * rng.standard_normal(N), -0.999, 0.999)
+    y = np.clip(y + 0.05 * (2 / 300) * rng.standard_normal(N), -0.999, 0.999)
+np.savez({path!r}, **out)
+"""
+
+
+def test_mesher_threads_do_not_change_the_mesh(tmp_path):
+    """The mesher's host threads split work in fixed chunks joined in order:
+    one thread and many give the same meshes, bit for bit."""
+    import os
+    import subprocess
+    import sys
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "huji-rich_amd")
+    res = {}
+    for t in ("1", "8"):
+        path = str(tmp_path / f"m{t}.npz")
+        env = dict(os.environ, RICH_MESH_THREADS=t)
+        subprocess.run([sys.executable, "-c", _SEQ.format(pkg=pkg, path=path)], env=env,
+                       check=True)
+        res[t] = np.load(path)
+    for k in res["1"].files:
+        np.testing.assert_array_equal(res["1"][k], res["8"][k], err_msg=k)
