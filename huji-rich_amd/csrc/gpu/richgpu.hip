@@ -16,6 +16,7 @@
 #include "adjoint.hpp"
 #include "ensemble.hpp"
 #include "moving.hpp"
+#include "../mesh/parallel.hpp"
 #include "kernels_mixed.hpp"
 
 using namespace rg;
@@ -743,19 +744,34 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     if (m->ghost_orig[g] < 0 || m->ghost_orig[g] >= N) return RG_E_ARG;
   auto orig = [&](int id) { return id < N ? id : m->ghost_orig[id - N]; };
   std::vector<Inc> inc(static_cast<size_t>(nnz));
-  for (int i = 0; i < N; ++i) {
-    int prev = -1;
-    for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-      const int f = m->csr_face[k];
-      if (f < 0 || f >= F || f <= prev) return RG_E_ARG;  // ascending face order
-      prev = f;
-      const int a = m->face_first[f], b = m->face_second[f];
-      if (a != i && b != i) return RG_E_ARG;
-      unsigned w = static_cast<unsigned>(f);
-      if (b == i) w |= INFO_SECOND;
-      if (orig(a) == orig(b)) w |= INFO_MIRROR;
-      inc[k] = Inc{(a == i) ? b : a, w};
-    }
+  {
+    // each cell's incidences on host threads; any bad entry fails the mesh
+    const int nc = rmp::chunks(N);
+    std::vector<char> bad(static_cast<size_t>(nc), 0);
+    rmp::for_chunks(N, nc, [&](int ch, int64_t i0, int64_t i1) {
+      for (int i = static_cast<int>(i0); i < static_cast<int>(i1); ++i) {
+        int prev = -1;
+        for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
+          const int f = m->csr_face[k];
+          if (f < 0 || f >= F || f <= prev) {  // ascending face order
+            bad[ch] = 1;
+            return;
+          }
+          prev = f;
+          const int a = m->face_first[f], b = m->face_second[f];
+          if (a != i && b != i) {
+            bad[ch] = 1;
+            return;
+          }
+          unsigned w = static_cast<unsigned>(f);
+          if (b == i) w |= INFO_SECOND;
+          if (orig(a) == orig(b)) w |= INFO_MIRROR;
+          inc[k] = Inc{(a == i) ? b : a, w};
+        }
+      }
+    });
+    for (char v : bad)
+      if (v) return RG_E_ARG;
   }
   // Ghost sources: RigidWallGenerator copies the real cell of the ghost's
   // first boundary face (its flat_map insert keeps the first) and reverses
@@ -841,24 +857,38 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   };
   std::vector<int> enbr(ne, 0);
   std::vector<unsigned> einfo(ne, 0u);
-  for (int i = 0; i < N; ++i)
+  rmp::parallel_for(N, [&](int64_t ii) {
+    const int i = static_cast<int>(ii);
     for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
       enbr[ell_at(i, k)] = inc[k].nbr;
       einfo[ell_at(i, k)] = inc[k].info;
     }
+  });
   // ELL entry of each face on its first [2f] and second [2f + 1] side; every
   // face between two real cells must be in both cells' lists
   std::vector<int> epos(2 * static_cast<size_t>(F), -1);
-  for (int i = 0; i < N; ++i)
+  rmp::parallel_for(N, [&](int64_t ii) {  // each (face, side) has one cell
+    const int i = static_cast<int>(ii);
     for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
       const unsigned w = inc[k].info;
       epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 1 : 0)] =
           static_cast<int>(ell_at(i, k));
     }
-  for (int k = 0; k < nnz; ++k)
-    if (inc[k].nbr < N &&
-        epos[2 * static_cast<size_t>(inc[k].info & INFO_FACE) + ((inc[k].info & INFO_SECOND) ? 0 : 1)] < 0)
-      return RG_E_ARG;  // face missing from the neighbour's list
+  });
+  {
+    const int nc = rmp::chunks(nnz);
+    std::vector<char> bad(static_cast<size_t>(nc), 0);
+    rmp::for_chunks(nnz, nc, [&](int ch, int64_t k0, int64_t k1) {
+      for (int64_t k = k0; k < k1; ++k)
+        if (inc[k].nbr < N && epos[2 * static_cast<size_t>(inc[k].info & INFO_FACE) +
+                                   ((inc[k].info & INFO_SECOND) ? 0 : 1)] < 0) {
+          bad[ch] = 1;  // face missing from the neighbour's list
+          return;
+        }
+    });
+    for (char v : bad)
+      if (v) return RG_E_ARG;
+  }
   int *wofs_d, *enbr_d;
   unsigned* einfo_d;
   D2 *eh, *eo;
