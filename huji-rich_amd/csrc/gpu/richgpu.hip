@@ -113,9 +113,20 @@ struct rg_ctx {
   int adj_steps = 0, adj_every = 0, adj_nck = 0, adj_cap_every = 0, adj_cap_nck = 0;
   std::vector<void*> adj_bufs;  // checkpoints and tape (sized per run)
   D4 *ckP = nullptr, *ckX = nullptr, *finP = nullptr, *finX = nullptr;
-  D4 *tP = nullptr, *tX = nullptr, *tPm = nullptr, *tXm = nullptr;
-  Rec *tR1 = nullptr, *tR2 = nullptr;
-  double* tdt = nullptr;
+  // two tapes of a segment's recompute (states, mid states, both stages'
+  // slopes, dts): the recompute of one segment fills one while the reverse
+  // sweep of the next reads the other
+  struct Tape {
+    D4 *P = nullptr, *X = nullptr, *Pm = nullptr, *Xm = nullptr;
+    Rec *R1 = nullptr, *R2 = nullptr;
+    double* dt = nullptr;
+  } tape[2];
+  // the recompute's own stream, fluxes, dt and error words and block minima
+  hipStream_t rstream = nullptr;
+  D4* rflux = nullptr;
+  unsigned long long* rwords = nullptr;
+  double* rpartials = nullptr;
+  hipEvent_t tape_ready[2] = {nullptr, nullptr}, tape_free[2] = {nullptr, nullptr};
   D4 *gP = nullptr, *gX = nullptr, *ga = nullptr, *gb = nullptr, *gps = nullptr;
   D4 *lb = nullptr, *rb = nullptr, *emit = nullptr;
   double *dtb = nullptr, *vmin = nullptr;
@@ -231,9 +242,11 @@ void free_mesh(rg_ctx* c, bool pool = false) {
   c->am = AdjMesh{};
   c->adj_src = rg_ctx::AdjSource{};
   c->adj_steps = c->adj_every = c->adj_nck = c->adj_cap_every = c->adj_cap_nck = 0;
-  c->ckP = c->ckX = c->finP = c->finX = c->tP = c->tX = c->tPm = c->tXm = nullptr;
-  c->tR1 = c->tR2 = nullptr;
-  c->tdt = nullptr;
+  c->ckP = c->ckX = c->finP = c->finX = nullptr;
+  c->tape[0] = c->tape[1] = rg_ctx::Tape{};
+  c->rflux = nullptr;
+  c->rwords = nullptr;
+  c->rpartials = nullptr;
   c->gP = c->gX = c->ga = c->gb = c->gps = c->lb = c->rb = c->emit = nullptr;
   c->dtb = c->vmin = nullptr;
   c->imin = nullptr;
@@ -702,6 +715,14 @@ void rg_destroy(rg_ctx* c) {
     (void)hipEventDestroy(t.b);
   }
   for (hipEvent_t ev : c->event_pool) (void)hipEventDestroy(ev);
+  for (int b = 0; b < 2; ++b) {
+    if (c->tape_ready[b]) (void)hipEventDestroy(c->tape_ready[b]);
+    if (c->tape_free[b]) (void)hipEventDestroy(c->tape_free[b]);
+  }
+  if (c->rstream) {
+    (void)hipStreamSynchronize(c->rstream);
+    (void)hipStreamDestroy(c->rstream);
+  }
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1297,13 +1318,18 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
     HIPCHK(c, get(&c->ckX, n * nck));
     HIPCHK(c, get(&c->finP, n));
     HIPCHK(c, get(&c->finX, n));
-    HIPCHK(c, get(&c->tP, n * (ev + 1)));
-    HIPCHK(c, get(&c->tX, n * (ev + 1)));
-    HIPCHK(c, get(&c->tPm, n * ev));
-    HIPCHK(c, get(&c->tXm, n * ev));
-    HIPCHK(c, get(&c->tR1, n * ev));
-    HIPCHK(c, get(&c->tR2, n * ev));
-    HIPCHK(c, get(&c->tdt, static_cast<size_t>(ev)));
+    for (rg_ctx::Tape& T : c->tape) {
+      HIPCHK(c, get(&T.P, n * (ev + 1)));
+      HIPCHK(c, get(&T.X, n * (ev + 1)));
+      HIPCHK(c, get(&T.Pm, n * ev));
+      HIPCHK(c, get(&T.Xm, n * ev));
+      HIPCHK(c, get(&T.R1, n * ev));
+      HIPCHK(c, get(&T.R2, n * ev));
+      HIPCHK(c, get(&T.dt, static_cast<size_t>(ev)));
+    }
+    HIPCHK(c, get(&c->rflux, static_cast<size_t>(F)));
+    HIPCHK(c, get(&c->rwords, 4));
+    HIPCHK(c, get(&c->rpartials, static_cast<size_t>(blocks_for(std::max(N, F)))));
     HIPCHK(c, get(&c->gP, n));
     HIPCHK(c, get(&c->gX, n));
     HIPCHK(c, get(&c->ga, n));
@@ -1317,6 +1343,13 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
     HIPCHK(c, get(&c->imin, static_cast<size_t>(blocks_for(N))));
     c->adj_cap_nck = nck;
     c->adj_cap_every = ev;
+  }
+  if (!c->rstream) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->rstream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(c, hipEventCreateWithFlags(&c->tape_ready[b], hipEventDisableTiming));
+      HIPCHK(c, hipEventCreateWithFlags(&c->tape_free[b], hipEventDisableTiming));
+    }
   }
   if (dt_out && c->dt_hist_cap < nsteps) {
     if (c->dt_hist) (void)hipFree(c->dt_hist);
@@ -1397,33 +1430,67 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
   }
   PendingDt pd{nullptr, c->dtb, static_cast<int>(2 * nbF), c->vmin, c->imin,
                static_cast<int>(nbN), c->prm.cfl};
+  // The recompute runs on its own stream with its own fluxes, dt and error
+  // words and block minima: while the enqueue below is inside a Recompute
+  // scope, the context's stream and those buffers are the recompute's.
+  struct Recompute {
+    rg_ctx* c;
+    explicit Recompute(rg_ctx* cc) : c(cc) { flip(); }
+    ~Recompute() { flip(); }
+    void flip() {
+      std::swap(c->stream, c->rstream);
+      std::swap(c->flux, c->rflux);
+      std::swap(c->words, c->rwords);
+      std::swap(c->partials, c->rpartials);
+    }
+  };
+  {
+    Recompute r(c);
+    HIPCHK(c, fill_words(c, 2, 1, ~0ull));  // no error yet
+  }
+  for (int b = 0; b < 2; ++b) HIPCHK(c, hipEventRecord(c->tape_free[b], c->stream));
+  // recompute a segment from its checkpoint into tape seg & 1: states
+  // P/X[0..L], mid states, both stages' slopes and the dts
+  auto recompute = [&](int seg) -> hipError_t {
+    Recompute r(c);
+    const rg_ctx::Tape& T = c->tape[seg & 1];
+    const int k0 = seg * every, L = std::min(k0 + every, nsteps) - k0;
+    hipError_t e = hipStreamWaitEvent(c->stream, c->tape_free[seg & 1], 0);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(T.P, c->ckP + static_cast<size_t>(seg) * n, bytes,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(T.X, c->ckX + static_cast<size_t>(seg) * n, bytes,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = run_cfl(c, T.P, k0 & 1);
+    for (int t = 0; t < L && e == hipSuccess; ++t) {
+      const size_t o = static_cast<size_t>(t) * n;
+      const HeunBufs h{T.P + o,     T.X + o,     T.Pm + o, T.Xm + o,
+                       T.P + o + n, T.X + o + n, T.R1 + o, T.R2 + o};
+      e = heun_step(c, k0 + t, T.dt, t, h);
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->tape_ready[seg & 1], c->stream);
+    return e;
+  };
+  HIPCHK(c, recompute(c->adj_nck - 1));
   for (int seg = c->adj_nck - 1; seg >= 0; --seg) {
     const int k0 = seg * every, k1 = std::min(k0 + every, nsteps), L = k1 - k0;
-    // recompute the segment from its checkpoint into the tape: states
-    // tP/tX[0..L], mid states, both stages' slopes and the dts
-    HIPCHK(c, hipMemcpyAsync(c->tP, c->ckP + static_cast<size_t>(seg) * n, bytes,
-                             hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->tX, c->ckX + static_cast<size_t>(seg) * n, bytes,
-                             hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, run_cfl(c, c->tP, k0 & 1));
-    for (int t = 0; t < L; ++t) {
-      const size_t o = static_cast<size_t>(t) * n;
-      const HeunBufs h{c->tP + o,  c->tX + o,     c->tPm + o,    c->tXm + o,
-                       c->tP + o + n, c->tX + o + n, c->tR1 + o, c->tR2 + o};
-      HIPCHK(c, heun_step(c, k0 + t, c->tdt, t, h));
-    }
+    if (seg > 0) HIPCHK(c, recompute(seg - 1));  // beside this segment's reverse
+    const rg_ctx::Tape& T = c->tape[seg & 1];
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->tape_ready[seg & 1], 0));
     // reverse through the segment
     for (int t = L - 1; t >= 0; --t) {
       const size_t o = static_cast<size_t>(t) * n;
       // the dt adjoint of step k0+t+1 is pending except for the last step
-      pd.q = (k0 + t + 1 < nsteps) ? c->tP + o + n : nullptr;
+      pd.q = (k0 + t + 1 < nsteps) ? T.P + o + n : nullptr;
       HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
-        k_adj_recover<<<nbN, kBlock, 0, c->stream>>>(c->dm, c->tX + o + n, c->gP, c->gX, c->ga,
+        k_adj_recover<<<nbN, kBlock, 0, c->stream>>>(c->dm, T.X + o + n, c->gP, c->gX, c->ga,
                                                      c->gb, c->prm.gamma, pd);
       }));
-      HIPCHK(c, adj_stage(c, c->tPm + o, c->tR2 + o, c->tXm + o, c->tdt + t, c->dtb, 2));
-      HIPCHK(c, adj_stage(c, c->tP + o, c->tR1 + o, nullptr, c->tdt + t, c->dtb + nbF, 1));
+      HIPCHK(c, adj_stage(c, T.Pm + o, T.R2 + o, T.Xm + o, T.dt + t, c->dtb, 2));
+      HIPCHK(c, adj_stage(c, T.P + o, T.R1 + o, nullptr, T.dt + t, c->dtb + nbF, 1));
     }
+    HIPCHK(c, hipEventRecord(c->tape_free[seg & 1], c->stream));
   }
   pd.q = c->ckP;  // the dt adjoint of step 0
   HIPCHK(c, launch(c, KID_ADJ_CELL, [&] {
@@ -1437,7 +1504,13 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
   // leave the forward run's final state in place
   HIPCHK(c, hipMemcpyAsync(c->prim, c->finP, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->ext, c->finX, bytes, hipMemcpyDeviceToDevice, c->stream));
-  return collect_error(c);
+  int rc;
+  {
+    Recompute r(c);  // errors the recompute reported
+    rc = collect_error(c);
+  }
+  const int rc0 = collect_error(c);
+  return rc != RG_OK ? rc : rc0;
 }
 
 int rg_set_instances(rg_ctx* c, int32_t n_inst, const int32_t* cell_off) {
