@@ -150,9 +150,12 @@ struct rg_ctx {
     std::vector<int> epos;       // [2F] ELL entry of each face's first and second side
   } adj_src;
 
-  // moving mesh: point velocities kept across rg_replace_mesh
+  // moving mesh: point velocities kept across rg_replace_mesh, and where
+  // the state waits while the mesh is swapped
   D2* pv = nullptr;
   int pv_cap = 0;
+  D4* keep = nullptr;
+  size_t keep_cap = 0;
   bool pv_valid = false;
 
   bool prof = false;
@@ -707,6 +710,7 @@ void rg_destroy(rg_ctx* c) {
   free_mesh(c);
   drain_pool(c);
   if (c->pv) (void)hipFree(c->pv);
+  if (c->keep) (void)hipFree(c->keep);
   if (c->dt_hist) (void)hipFree(c->dt_hist);
   if (c->words) (void)hipFree(c->words);
   if (c->host_words) (void)hipHostFree(c->host_words);
@@ -1646,8 +1650,14 @@ int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {
   if (!c || !m || !c->has_mesh || !c->has_cells || !c->has_ext || m->n_cells != c->N)
     return RG_E_ARG;
   const size_t bytes = sizeof(D4) * static_cast<size_t>(c->N);
-  D4* keep = nullptr;  // cells, extensives, mid extensives
-  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&keep), 3 * bytes));
+  if (c->keep_cap < 3 * static_cast<size_t>(c->N)) {  // cells, extensives, mid extensives
+    if (c->keep) (void)hipFree(c->keep);
+    c->keep = nullptr;
+    c->keep_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->keep), 3 * bytes));
+    c->keep_cap = 3 * static_cast<size_t>(c->N);
+  }
+  D4* keep = c->keep;
   hipError_t e = hipMemcpyAsync(keep, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(keep + c->N, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream);
@@ -1655,10 +1665,7 @@ int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {
     e = hipMemcpyAsync(keep + 2 * static_cast<size_t>(c->N), c->mid_ext, bytes,
                        hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) {
-    (void)hipFree(keep);
-    return fail_hip(c, e);
-  }
+  if (e != hipSuccess) return fail_hip(c, e);
   const bool pv_valid = c->pv_valid;
   const int rc = rg_set_mesh(c, m);
   if (rc == RG_OK) {
@@ -1670,7 +1677,6 @@ int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {
                          hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   }
-  (void)hipFree(keep);
   if (rc != RG_OK) return rc;
   if (e != hipSuccess) return fail_hip(c, e);
   c->has_cells = c->has_ext = true;
