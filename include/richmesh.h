@@ -75,7 +75,9 @@ typedef struct rm_points_params {
 
 rm_mesh* rm_points_build(const rm_points_params* p);
 
-/* The mesh of points in a rigid-wall box in one call: ghosts are the
+/* The mesh of points in a rigid-wall box in one call (what
+ * VoronoiMesh::Initialise, VoronoiMesh.cpp:333-456, builds for a SquareBox):
+ * ghosts are the
  * reflections across each wall of the points within `margin` of it (walls
  * in the order 0 right, 1 up, 2 left, 3 down, points in index order), the
  * triangulation is rm_delaunay's, and the triangles with a real vertex and
@@ -90,8 +92,9 @@ typedef struct rm_box_points_params {
 
 rm_mesh* rm_box_points_build(const rm_box_points_params* p);
 
-/* A moving mesh's rebuilds: rm_mover_mesh gives the mesh of the points'
- * current positions (n, box and margin of rm_mover_create; its x, y are not
+/* A moving mesh's rebuilds (VoronoiMesh::Update, VoronoiMesh.cpp:458-571,
+ * which hdsim calls from MoveMeshPoints every step): rm_mover_mesh gives the
+ * mesh of the points' current positions (n, box and margin of rm_mover_create; its x, y are not
  * read), the same mesh rm_box_points_build gives, up to the diagonal of
  * cocircular point quads.  It keeps the previous triangulation of points and
  * ghosts and, when the moved points leave every triangle counter-clockwise,
