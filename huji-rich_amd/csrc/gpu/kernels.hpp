@@ -60,8 +60,7 @@ struct alignas(8) Inc {
   int nbr;
   unsigned info;
 };
-struct alignas(32) Rec {  // a cell and its limited slopes
-  double d, p, vx, vy;
+struct alignas(64) Rec {  // a cell's limited slopes
   double xd, xp, xvx, xvy;
   double yd, yp, yvx, yvy;
 };
@@ -374,7 +373,7 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
   Cell amax = cell, amin = cell, bmax = cell, bmin = cell;
   if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
     report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
-    out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0};
+    out[i] = Rec{0, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
   if (kind == KIND_BOUNDARY) {
@@ -494,8 +493,7 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
     sx.vy *= psi3;
     sy.vy *= psi3;
   }
-  out[i] = Rec{cell.d, cell.p, cell.vx, cell.vy, sx.d, sx.p, sx.vx, sx.vy,
-               sy.d,   sy.p,   sy.vx,   sy.vy};
+  out[i] = Rec{sx.d, sx.p, sx.vx, sx.vy, sy.d, sy.p, sy.vx, sy.vy};
 }
 
 __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
@@ -517,8 +515,9 @@ __global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict_
 __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Rec* rec, int id,
                                            double ox, double oy) {
   if (id < m.N || m.bc == BC_PERIODIC) {
-    const Rec r = rec[id < m.N ? id : m.gorig[id - m.N]];
-    const Cell v{r.d, r.p, r.vx, r.vy};
+    const int o = id < m.N ? id : m.gorig[id - m.N];
+    const Rec r = rec[o];
+    const Cell v = ld_cell(q, o);
     const Cell sx{r.xd, r.xp, r.xvx, r.xvy}, sy{r.yd, r.yp, r.yvx, r.yvy};
     return interp_off(v, sx, sy, ox, oy);
   }

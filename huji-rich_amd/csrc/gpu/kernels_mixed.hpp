@@ -29,8 +29,8 @@ struct alignas(32) CellGeoM {
   float kind, width, c0, c1, c2, c3, pad0, pad1;
 };
 // per cell record: cell, x-slope, y-slope
-struct alignas(16) RecM {
-  float d, p, vx, vy, xd, xp, xvx, xvy, yd, yp, yvx, yvy;
+struct alignas(32) RecM {  // a cell's limited slopes in FP32
+  float xd, xp, xvx, xvy, yd, yp, yvx, yvy;
 };
 
 // Incidences in wavefront-sliced ELL order: the cells of wavefront w are
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
   CellF amax = cell, amin = cell, bmax = cell, bmin = cell;
   if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
     report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
-    out[i] = RecM{cell.d, cell.p, cell.vx, cell.vy, 0, 0, 0, 0, 0, 0, 0, 0};
+    out[i] = RecM{0, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
   if (kind == KIND_BOUNDARY) {
@@ -272,16 +272,16 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
     sx.vy *= psi3;
     sy.vy *= psi3;
   }
-  out[i] = RecM{cell.d, cell.p, cell.vx, cell.vy, sx.d, sx.p, sx.vx, sx.vy,
-                sy.d,   sy.p,   sy.vx,   sy.vy};
+  out[i] = RecM{sx.d, sx.p, sx.vx, sx.vy, sy.d, sy.p, sy.vx, sy.vy};
 }
 
 // face state of one side in FP32; (ox, oy) = centroid - CM of that side
 __device__ __forceinline__ CellF side_state_m(const DMesh& m, const F4* q, const RecM* rec, int id,
                                               float ox, float oy) {
   if (id >= m.N && m.bc == BC_RIGID) return fetchf(m, q, id);  // zero ghost gradient
-  const RecM r = rec[id < m.N ? id : m.gorig[id - m.N]];
-  CellF v{r.d, r.p, r.vx, r.vy};
+  const int o = id < m.N ? id : m.gorig[id - m.N];
+  const RecM r = rec[o];
+  CellF v = ld_cellf(q, o);
   add_multf(v, CellF{r.xd, r.xp, r.xvx, r.xvy}, ox);
   add_multf(v, CellF{r.yd, r.yp, r.yvx, r.yvy}, oy);
   return v;
