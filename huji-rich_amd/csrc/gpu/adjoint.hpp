@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D
   cell_state_adj(m, am, lbf, rbf, i, deg, eb, vstate, gx, gy);
   const Cell cell = ld_cell(q, i);
   const CellGeo g = m.cgeo[i];
-  const int kind = static_cast<int>(g.kind);
+  const int kind = g.kind;
+  const D2 cmi = m.cm[i];
   if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
     for (int s = 0; s < deg; ++s) emit[eb + 64 * s] = D4{0, 0, 0, 0};
     gps[i] = cell_d4(vstate);
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D
     const Cell nc = fetch(m, q, nb);
     if (boundary) {
       const D2 cj = m.cm[nb];
-      const double ey = cj.y - g.cmy, ex = cj.x - g.cmx;
+      const double ey = cj.y - cmi.y, ex = cj.x - cmi.x;
       const Cell dd = sub(nc, cell);
       phy.d += dd.d * ey;
       phy.p += dd.p * ey;
@@ -411,8 +412,8 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D
     double ax, ay;
     if (boundary) {
       const D2 cj = m.cm[m.enbr[eb + 64 * s]];
-      ax = cj.x - g.cmx;
-      ay = cj.y - g.cmy;
+      ax = cj.x - cmi.x;
+      ay = cj.y - cmi.y;
     } else {
       const D2 h = m.eh[eb + 64 * s];
       ax = h.x;

@@ -44,9 +44,10 @@ struct alignas(16) D2 {
 struct alignas(32) D4 {  // cell {d,p,vx,vy}; extensive / flux {m,E,Mx,My}
   double a, b, c, d;
 };
-struct alignas(64) CellGeo {
-  double cmx, cmy, width, kind;
+struct alignas(16) CellGeo {
   double c0, c1, c2, c3;  // naive: inverse matrix i0..i3; boundary: SxSy, Sx2, Sy2, den
+  double width;
+  int kind, deg;          // slope kind, number of faces
 };
 struct alignas(32) FaceRec {
   double hx, hy, cx, cy;
@@ -234,9 +235,8 @@ __global__ void __launch_bounds__(256) k_geo_cells(DMesh m, CellGeo* cg, double*
   const int b = m.off[i], e = m.off[i + 1], n = e - b;
   const D2 cmi = m.cm[i];
   CellGeo g{};
-  g.cmx = cmi.x;
-  g.cmy = cmi.y;
   g.width = sqrt(m.tvol[i] / PI);
+  g.deg = n;
   bool boundary = false;
   for (int k = b; k < e; ++k)
     if (m.inc[k].info & INFO_MIRROR) boundary = true;
@@ -363,11 +363,10 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
                                             Rec* __restrict__ out, const DParams& prm,
                                             const DErr& err, int i) {
   if (i >= m.N) return;
-  const int b = m.off[i], e = m.off[i + 1], deg = e - b;
+  const CellGeo g = m.cgeo[i];
+  const int kind = g.kind, deg = g.deg;
   const int eb = ell_base(m, i);
   const Cell cell = ld_cell(q, i);
-  const CellGeo g = m.cgeo[i];
-  const int kind = static_cast<int>(g.kind);
   Cell sx{0, 0, 0, 0}, sy{0, 0, 0, 0};
   double pr = 1;
   Cell amax = cell, amin = cell, bmax = cell, bmin = cell;
@@ -379,11 +378,13 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
   if (kind == KIND_BOUNDARY) {
     // GetBoundarySlope (LinearGaussImproved.cpp:401-422)
     Cell phy{0, 0, 0, 0}, phx{0, 0, 0, 0};
+    const D2 cmi = m.cm[i];
+    const int b = m.off[i], e = b + deg;
     for (int k = b; k < e; ++k) {
       const Inc ic = m.inc[k];
       const Cell nc = fetch(m, q, ic.nbr);
       const D2 cj = m.cm[ic.nbr];
-      const double ey = cj.y - g.cmy, ex = cj.x - g.cmx;
+      const double ey = cj.y - cmi.y, ex = cj.x - cmi.x;
       const Cell dd = sub(nc, cell);
       phy.d += dd.d * ey;
       phy.p += dd.p * ey;
@@ -546,7 +547,7 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
   if (rc) {
     report(err, f, rc);
   } else if (dt > 0) {
-    const double sc = dt * m.area[f];
+    const double sc = dt * ap;  // the face area |v1 - v2| (physical_geometry.cpp:8-11)
     o = D4{c.m * sc, c.E * sc, c.mx * sc, c.my * sc};
   } else {
     o = D4{c.m, c.E, c.mx, c.my};
