@@ -140,8 +140,14 @@ def test_adjoint_sedov_256x256_dot_product():
 
 def test_adjoint_256x256_matches_finite_differences_without_limiter():
     """Where the step is differentiable (limiter off, smooth state), central
-    differences (h = 1e-6) of full-size GPU runs agree with the adjoint to
-    1e-6 relative; 40 steps, checkpoint every 20."""
+    differences of full-size GPU runs agree with the adjoint to 1e-6
+    relative; 40 steps, checkpoint every 20.  The difference is taken cell by
+    cell before the weighted sum, so that the sum does not cancel two large
+    nearly equal losses.  Even so the step is only piecewise smooth: the CFL
+    dt is a minimum over 65k cells (near ties switch its argmin) and the HLLC
+    wave pattern switches where a velocity crosses a wave speed, so one step
+    size may straddle a kink.  Each direction is differenced with h = 1e-6
+    and 1e-7 and the closer of the two must agree."""
     from rich_amd.gpu import RichGPU
     from test_oracle_tangent import smooth_cells
     m = cases.sedov_mesh(256)
@@ -155,19 +161,22 @@ def test_adjoint_256x256_matches_finite_differences_without_limiter():
     gc0 = g.adjoint_backward(w)
     g.close()
 
-    def loss(c):
+    def run(c):
         gg = RichGPU(m, order=2, slf=False)
         gg.set_cells(c)
         gg.init_extensives()
         gg.advance(40, integrator=2, record_dt=False)
-        out = float(np.sum(w * gg.get_cells()))
+        out = gg.get_cells()
         gg.close()
         return out
 
     for k in range(3):
         v = rng.standard_normal((4, m.n_cells)) * np.abs(cells)
-        h = 1e-6
-        fd = (loss(cells + h * v) - loss(cells - h * v)) / (2 * h)
         an = float(np.sum(gc0 * v))
-        print(f"direction {k}: adjoint {an:.12e} central difference {fd:.12e}")
-        assert abs(fd - an) <= 1e-6 * abs(an)
+        errs = []
+        for h in (1e-6, 1e-7):
+            fd = float(np.sum(w * (run(cells + h * v) - run(cells - h * v)))) / (2 * h)
+            errs.append(abs(fd - an) / abs(an))
+            print(f"direction {k} h {h:.0e}: adjoint {an:.12e} central difference {fd:.12e} "
+                  f"rel {errs[-1]:.2e}")
+        assert min(errs) <= 1e-6
