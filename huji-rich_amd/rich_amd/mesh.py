@@ -130,20 +130,23 @@ def lattice(nx, ny, box=(0.0, 1.0, 0.0, 1.0), jitter=1e-3, seed=0, bc=RIGID, fli
     return _export(h, bc, lattice_ij=True)
 
 
+_DELAUNAY_ERR = {-1: "bad input", -2: "no hull point found", -3: "a duplicate point, or one "
+                 "inside the hull", -4: "flip budget spent", -5: "too many triangles",
+                 -6: "a triangle turned clockwise", -7: "the hull is no longer convex"}
+
+
 def delaunay(x, y):
-    """(T, 3) counter-clockwise Delaunay triangles of the points: rm_delaunay,
-    or Qhull (scipy.spatial.Delaunay) where rm_delaunay reports an input it
-    cannot triangulate (duplicate points, a point its sweep cannot place)."""
+    """(T, 3) counter-clockwise Delaunay triangles of the points (rm_delaunay);
+    ValueError for input it cannot triangulate (e.g. duplicate points)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     n = len(x)
     tri = np.empty((2 * n, 3), dtype=np.int32)
     nt = ctypes.c_int32()
     rc = _lib().rm_delaunay(n, _ffi.ptr(x), _ffi.ptr(y), _ffi.ptr(tri), 2 * n, ctypes.byref(nt))
-    if rc == 0:
-        return tri[:nt.value]
-    from scipy.spatial import Delaunay
-    return Delaunay(np.column_stack([x, y])).simplices.astype(np.int32)
+    if rc != 0:
+        raise ValueError(f"rm_delaunay: {_DELAUNAY_ERR.get(rc, rc)}")
+    return tri[:nt.value]
 
 
 def points(x, y, box, margin=None):
@@ -152,9 +155,8 @@ def points(x, y, box, margin=None):
     wall of the points within `margin` of it (default: 3 mean spacings), as
     the lattice builder reflects its outer rows; rm_box_points_build
     triangulates points and ghosts and forms faces and geometry by the
-    lattice builder's rules.  Where it cannot triangulate the points, Qhull
-    (scipy.spatial.Delaunay) does and rm_points_build forms the mesh.  Cells
-    keep the order of the points."""
+    lattice builder's rules.  Cells keep the order of the points.
+    ValueError for points it cannot triangulate (e.g. duplicates)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     N = len(x)
@@ -163,9 +165,9 @@ def points(x, y, box, margin=None):
         margin = 3.0 * np.sqrt((x1 - x0) * (y1 - y0) / N)
     bp = _BoxPointsParams(N, _ffi.ptr(x), _ffi.ptr(y), x0, x1, y0, y1, margin)
     h = _lib().rm_box_points_build(ctypes.byref(bp))
-    if h:
-        return _export(h, RIGID, lattice_ij=False)
-    return _points_qhull(x, y, box, margin)
+    if not h:
+        raise ValueError("rm_box_points_build could not triangulate the points")
+    return _export(h, RIGID, lattice_ij=False)
 
 
 class BoxMesher:
@@ -190,9 +192,9 @@ class BoxMesher:
         if x.shape != (self.n,) or y.shape != (self.n,):
             raise ValueError(f"expected {self.n} points")
         h = self._lib.rm_mover_mesh(self._h, _ffi.ptr(x), _ffi.ptr(y))
-        if h:
-            return _export(h, RIGID, lattice_ij=False)
-        return _points_qhull(x, y, self.box, self.margin)
+        if not h:
+            raise ValueError("rm_mover_mesh could not triangulate the points")
+        return _export(h, RIGID, lattice_ij=False)
 
     def counts(self):
         """(full triangulations, flip repairs) so far."""
@@ -212,38 +214,19 @@ class BoxMesher:
             pass
 
 
-def _points_qhull(x, y, box, margin):
-    from scipy.spatial import Delaunay
-    N = len(x)
-    x0, x1, y0, y1 = box
-    gx, gy, go, gw = [], [], [], []
-    for wall, near, fx, fy in (
-            (0, x1 - x < margin, lambda: x + 2 * (x1 - x), lambda: y + 0.0),
-            (1, y1 - y < margin, lambda: x + 0.0, lambda: y + 2 * (y1 - y)),
-            (2, x - x0 < margin, lambda: x + 2 * (x0 - x), lambda: y + 0.0),
-            (3, y - y0 < margin, lambda: x + 0.0, lambda: y + 2 * (y0 - y))):
-        idx = np.nonzero(near)[0]
-        gx.append(fx()[idx])
-        gy.append(fy()[idx])
-        go.append(idx.astype(np.int32))
-        gw.append(np.full(len(idx), wall, np.int32))
-    X = np.concatenate([x] + gx)
-    Y = np.concatenate([y] + gy)
-    gorig = np.ascontiguousarray(np.concatenate(go), dtype=np.int32)
-    gwall = np.ascontiguousarray(np.concatenate(gw), dtype=np.int32)
-    tri = Delaunay(np.column_stack([X, Y])).simplices.astype(np.int64)
-    # keep the triangles with a real vertex and the ghosts they use
-    tri = tri[(tri < N).any(axis=1)]
-    used = np.unique(tri[tri >= N]) - N
-    new_id = np.full(len(gorig), -1, np.int64)
-    new_id[used] = N + np.arange(len(used))
-    tri = np.where(tri >= N, new_id[np.maximum(tri - N, 0)], tri)
-    X = np.ascontiguousarray(np.concatenate([x, X[N:][used]]))
-    Y = np.ascontiguousarray(np.concatenate([y, Y[N:][used]]))
-    gorig = np.ascontiguousarray(gorig[used])
-    gwall = np.ascontiguousarray(gwall[used])
+def from_triangles(x, y, ghost_orig, ghost_wall, tri, box):
+    """The mesh of real points [0, N) and their wall reflections [N, N+G)
+    from a triangulation the caller made (rm_points_build): x, y [N+G],
+    ghost_orig and ghost_wall [G] (0 right, 1 up, 2 left, 3 down), tri
+    (T, 3) point ids of the triangles with a real vertex."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    go = np.ascontiguousarray(ghost_orig, dtype=np.int32)
+    gw = np.ascontiguousarray(ghost_wall, dtype=np.int32)
     tri = np.ascontiguousarray(tri, dtype=np.int32)
-    p = _PointsParams(N, len(gorig), _ffi.ptr(X), _ffi.ptr(Y), _ffi.ptr(gorig), _ffi.ptr(gwall),
+    N = len(x) - len(go)
+    x0, x1, y0, y1 = box
+    p = _PointsParams(N, len(go), _ffi.ptr(x), _ffi.ptr(y), _ffi.ptr(go), _ffi.ptr(gw),
                       len(tri), _ffi.ptr(tri), x0, x1, y0, y1)
     h = _lib().rm_points_build(ctypes.byref(p))
     if not h:

@@ -14,6 +14,38 @@ from rich_amd import mesh as rmesh
 BOX = (-1.0, 1.0, -1.0, 1.0)
 
 
+def qhull_points_mesh(x, y, box, margin):
+    """The same ghosts as rich_amd.mesh.points, triangulated by Qhull
+    (scipy.spatial.Delaunay) instead, then rm_points_build: an independent
+    triangulation to compare the native one with."""
+    from scipy.spatial import Delaunay
+    N = len(x)
+    x0, x1, y0, y1 = box
+    gx, gy, go, gw = [], [], [], []
+    for wall, near, fx, fy in (
+            (0, x1 - x < margin, lambda: x + 2 * (x1 - x), lambda: y + 0.0),
+            (1, y1 - y < margin, lambda: x + 0.0, lambda: y + 2 * (y1 - y)),
+            (2, x - x0 < margin, lambda: x + 2 * (x0 - x), lambda: y + 0.0),
+            (3, y - y0 < margin, lambda: x + 0.0, lambda: y + 2 * (y0 - y))):
+        idx = np.nonzero(near)[0]
+        gx.append(fx()[idx])
+        gy.append(fy()[idx])
+        go.append(idx)
+        gw.append(np.full(len(idx), wall))
+    X = np.concatenate([x] + gx)
+    Y = np.concatenate([y] + gy)
+    gorig, gwall = np.concatenate(go), np.concatenate(gw)
+    tri = Delaunay(np.column_stack([X, Y])).simplices.astype(np.int64)
+    tri = tri[(tri < N).any(axis=1)]  # triangles with a real vertex
+    used = np.unique(tri[tri >= N]) - N
+    new_id = np.full(len(gorig), -1, np.int64)
+    new_id[used] = N + np.arange(len(used))
+    tri = np.where(tri >= N, new_id[np.maximum(tri - N, 0)], tri)
+    return rmesh.from_triangles(np.concatenate([x, X[N:][used]]),
+                                np.concatenate([y, Y[N:][used]]), gorig[used], gwall[used],
+                                tri, box)
+
+
 def _pairs(m):
     o = m.orig(np.stack([m.face_first, m.face_second]))
     return {tuple(sorted(p)) + (int(min(a, b) >= m.n_cells or max(a, b) >= m.n_cells),)
@@ -123,8 +155,10 @@ def test_delaunay_rejects_duplicates():
     nt = ctypes.c_int32()
     rc = lib.rm_delaunay(5, x.ctypes.data, y.ctypes.data, tri.ctypes.data, 10, ctypes.byref(nt))
     assert rc < 0
-    # the Python entry falls back on Qhull, which drops the duplicate
-    assert len(rmesh.delaunay(x, y)) == 2
+    with pytest.raises(ValueError):
+        rmesh.delaunay(x, y)
+    with pytest.raises(ValueError):
+        rmesh.points(np.concatenate([x, [0.5]]), np.concatenate([y, [0.5]]), (-0.5, 1.5, -0.5, 1.5))
 
 
 def _same_mesh(a, b):
@@ -145,7 +179,7 @@ def test_points_equals_qhull_path():
     x = np.clip(m0.pt_x[:N] + 0.2 * (2 / 30) * rng.uniform(-1, 1, N), -0.999, 0.999)
     y = np.clip(m0.pt_y[:N] + 0.2 * (2 / 30) * rng.uniform(-1, 1, N), -0.999, 0.999)
     a = rmesh.points(x, y, BOX)
-    b = rmesh._points_qhull(x, y, BOX, 3.0 * np.sqrt(4.0 / N))
+    b = qhull_points_mesh(x, y, BOX, 3.0 * np.sqrt(4.0 / N))
     _same_mesh(a, b)
 
 
