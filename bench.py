@@ -330,7 +330,9 @@ def main():
                     help="Heun steps of the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-profile", action="store_true",
-                    help="time without per-kernel HIP events")
+                    help="skip the per-kernel HIP-event run after the timed one")
+    ap.add_argument("--profile-steps", type=int, default=50,
+                    help="steps of the per-kernel HIP-event run that follows the timed run")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision run (FP32 reconstruction and HLLC)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
@@ -402,16 +404,26 @@ def main():
 
     log(f"fp64: {args.warmup} warm-up + {args.steps} timed steps")
     g.advance(args.warmup, integrator=2, record_dt=False)
-    if not args.no_profile:
-        g.profile(True)
     t0 = time.perf_counter()
     dts = g.advance(args.steps, integrator=2, record_dt=True)
     elapsed = time.perf_counter() - t0
-    prof = g.profile_read() if not args.no_profile else {}
     ext = g.get_extensives()
 
     value = N * args.steps / elapsed / 1e6
     c64 = g.get_cells()
+
+    def kernel_times(ctx):
+        """Per-kernel HIP-event averages over further steps (the timed run
+        above records no events)."""
+        if args.no_profile or args.profile_steps <= 0:
+            return {}
+        ctx.profile(True)
+        ctx.advance(args.profile_steps, integrator=2, record_dt=False)
+        out = ctx.profile_read()
+        ctx.profile(False)
+        return out
+
+    prof = kernel_times(g)
 
     # Mixed precision: the same run (same initial state, warm-up and steps)
     # with FP32 reconstruction and HLLC and FP64 extensives; L1 error of each
@@ -424,13 +436,11 @@ def main():
         gm.set_cells(cells)
         gm.init_extensives()
         gm.advance(args.warmup, integrator=2, record_dt=False)
-        if not args.no_profile:
-            gm.profile(True)
         tm0 = time.perf_counter()
         gm.advance(args.steps, integrator=2, record_dt=False)
         elapsed_m = time.perf_counter() - tm0
-        prof_m = gm.profile_read() if not args.no_profile else {}
         cm = gm.get_cells()
+        prof_m = kernel_times(gm)
         vm = N * args.steps / elapsed_m / 1e6
         names = ("density", "pressure", "x_velocity", "y_velocity")
         l1 = {k: float(np.mean(np.abs(cm[j] - c64[j]))) for j, k in enumerate(names)}
