@@ -145,8 +145,8 @@ __global__ void __launch_bounds__(256) k_adj_flux(DMesh m, const D4* __restrict_
     const double dt = *dtp;
     const FaceFrame fr = m.fframe[f];
     const int a = m.ff[f], b = m.fs[f];
-    const Cell L = side_state(m, q, rec, a, fr.oxa, fr.oya);
-    const Cell R = side_state(m, q, rec, b, fr.oxb, fr.oyb);
+    Cell L, R;
+    face_states(m, q, rec, a, b, fr, L, R);
     const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
     Cons c;
     (void)hllc_face(L, R, gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, 0.0, c);

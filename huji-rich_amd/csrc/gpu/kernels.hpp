@@ -145,6 +145,41 @@ __device__ __forceinline__ Cell fetch(const DMesh& m, const D4* q, int id) {
   return r;
 }
 
+// fetch() of K point ids at once.  Ghost ids are resolved to their source
+// cells first (rare: only waves that touch the boundary take that branch),
+// then all K cell loads are issued back to back.  With fetch() called in a
+// loop, the ghost branch between two gathers makes the compiler wait for
+// the first gather before it issues the second.
+template <int K>
+__device__ __forceinline__ void fetch_k(const DMesh& m, const D4* q, const int (&id)[K],
+                                        Cell (&out)[K]) {
+  int src[K];
+  bool ghost = false;
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    src[u] = id[u];
+    ghost |= id[u] >= m.N;
+  }
+  if (ghost) {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (id[u] >= m.N) src[u] = m.gsrc[id[u] - m.N];
+  }
+#pragma unroll
+  for (int u = 0; u < K; ++u) out[u] = ld_cell(q, src[u]);
+  if (ghost && m.bc == BC_RIGID) {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (id[u] >= m.N) {
+        const D2 n = m.gn[id[u] - m.N];
+        Cell& r = out[u];
+        const double s2 = 2 * (r.vx * n.x + r.vy * n.y);
+        r.vx = r.vx - n.x * s2;
+        r.vy = r.vy - n.y * s2;
+      }
+  }
+}
+
 __device__ __forceinline__ Cell sub(const Cell& a, const Cell& b) {
   return Cell{a.d - b.d, a.p - b.p, a.vx - b.vx, a.vy - b.vy};
 }
@@ -363,18 +398,18 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
                                             Rec* __restrict__ out, const DParams& prm,
                                             const DErr& err, int i) {
   if (i >= m.N) return;
+  // A cell whose slope cannot be formed runs the loops with no faces and
+  // writes a zero record at the end: an early return here would let the
+  // compiler hold back every other load of the cell until its kind is known.
   const CellGeo g = m.cgeo[i];
-  const int kind = g.kind, deg = g.deg;
+  const int kind = g.kind;
+  const bool bad = kind == KIND_TOO_MANY || kind == KIND_SINGULAR;
+  const int deg = bad ? 0 : g.deg;
   const int eb = ell_base(m, i);
   const Cell cell = ld_cell(q, i);
   Cell sx{0, 0, 0, 0}, sy{0, 0, 0, 0};
   double pr = 1;
   Cell amax = cell, amin = cell, bmax = cell, bmin = cell;
-  if (kind == KIND_TOO_MANY || kind == KIND_SINGULAR) {
-    report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
-    out[i] = Rec{0, 0, 0, 0, 0, 0, 0, 0};
-    return;
-  }
   if (kind == KIND_BOUNDARY) {
     // GetBoundarySlope (LinearGaussImproved.cpp:401-422)
     Cell phy{0, 0, 0, 0}, phx{0, 0, 0, 0};
@@ -407,20 +442,18 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
     // right-hand side of calc_naive_slope, in face order with the cell's
     // own term first (LinearGaussImproved.cpp:91-104).  A cell on this path
     // has no mirror face, so one set of neighbour extrema serves both limiters.
+    // Slots past the cell's last face re-read its last slot (in bounds and
+    // already cached), so that no load sits behind a branch.
     Cell vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
     for (int s0 = 0; s0 < deg; s0 += CH) {
       int nb[CH];
       D2 h[CH];
       Cell nc[CH];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) {
-          nb[u] = m.enbr[eb + 64 * (s0 + u)];
-          h[u] = m.eh[eb + 64 * (s0 + u)];
-        }
+      for (int u = 0; u < CH; ++u) nb[u] = m.enbr[eb + 64 * min(s0 + u, deg - 1)];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) nc[u] = fetch(m, q, nb[u]);
+      for (int u = 0; u < CH; ++u) h[u] = m.eh[eb + 64 * min(s0 + u, deg - 1)];
+      fetch_k(m, q, nb, nc);
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         if (s0 + u >= deg) break;
@@ -446,6 +479,11 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
     add_mult(sx, vcy, i1);
     add_mult(sy, vcx, i2);
   }
+  if (bad) {
+    report(err, i, kind == KIND_TOO_MANY ? ERR_TOO_MANY_NEIGHBORS : ERR_SINGULAR_MATRIX);
+    out[i] = Rec{0, 0, 0, 0, 0, 0, 0, 0};
+    return;
+  }
   if (prm.slf) {
     // is_shock (LinearGaussImproved.cpp:150-157)
     const double cs = dp2c(prm.gamma, cell.d, cell.p);
@@ -458,8 +496,7 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
       for (int s0 = 0; s0 < deg; s0 += CH) {
         D2 o[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u)
-          if (s0 + u < deg) o[u] = m.eo[eb + 64 * (s0 + u)];
+        for (int u = 0; u < CH; ++u) o[u] = m.eo[eb + 64 * min(s0 + u, deg - 1)];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
           if (s0 + u >= deg) break;
@@ -474,15 +511,25 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
     } else {
       // shocked_slope_limit (LinearGaussImproved.cpp:302-399)
       const Cell maxd = sub(amax, cell), mind = sub(amin, cell);
-      for (int sl = 0; sl < deg; ++sl) {
-        const D2 o = m.eo[eb + 64 * sl];
-        const Cell nc = fetch(m, q, m.enbr[eb + 64 * sl]);
-        const Cell cv = interp_off(cell, sx, sy, o.x, o.y);
-        const Cell dphi = sub(cv, cell);
-        shocked_field(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, prm.theta, psi0);
-        shocked_field(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, prm.theta, psi1);
-        shocked_field(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, prm.theta, psi2);
-        shocked_field(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, prm.theta, psi3);
+      for (int s0 = 0; s0 < deg; s0 += CH) {
+        int nb[CH];
+        D2 o[CH];
+        Cell nc[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) nb[u] = m.enbr[eb + 64 * min(s0 + u, deg - 1)];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) o[u] = m.eo[eb + 64 * min(s0 + u, deg - 1)];
+        fetch_k(m, q, nb, nc);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          if (s0 + u >= deg) break;
+          const Cell cv = interp_off(cell, sx, sy, o[u].x, o[u].y);
+          const Cell dphi = sub(cv, cell);
+          shocked_field(dphi.d, cv.d, cell.d, nc[u].d, maxd.d, mind.d, prm.theta, psi0);
+          shocked_field(dphi.p, cv.p, cell.p, nc[u].p, maxd.p, mind.p, prm.theta, psi1);
+          shocked_field(dphi.vx, cv.vx, cell.vx, nc[u].vx, maxd.vx, mind.vx, prm.theta, psi2);
+          shocked_field(dphi.vy, cv.vy, cell.vy, nc[u].vy, maxd.vy, mind.vy, prm.theta, psi3);
+        }
       }
     }
     sx.d *= psi0;
@@ -526,6 +573,45 @@ __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Re
   return interp_off(fetch(m, q, id), z, z, ox, oy);
 }
 
+// side_state of both sides of face f, with the loads of both sides issued
+// together: ghost ids are resolved first (a branch only waves with a ghost
+// face take), then both slope records and cells are loaded.
+__device__ __forceinline__ void face_states(const DMesh& m, const D4* q, const Rec* rec, int a,
+                                            int b, const FaceFrame& fr, Cell& lf, Cell& rt) {
+  const int id[2] = {a, b};
+  int src[2] = {a, b};
+  const bool ghost = a >= m.N || b >= m.N;
+  if (ghost) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (id[u] >= m.N)
+        src[u] = m.bc == BC_PERIODIC ? m.gorig[id[u] - m.N] : m.gsrc[id[u] - m.N];
+  }
+  Rec r[2];
+  Cell v[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    r[u] = rec[src[u]];
+    v[u] = ld_cell(q, src[u]);
+  }
+  if (ghost && m.bc == BC_RIGID) {
+    // RigidWallGenerator: reflected copy, zero gradient
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (id[u] >= m.N) {
+        const D2 n = m.gn[id[u] - m.N];
+        const double s2 = 2 * (v[u].vx * n.x + v[u].vy * n.y);
+        v[u].vx = v[u].vx - n.x * s2;
+        v[u].vy = v[u].vy - n.y * s2;
+        r[u] = Rec{0, 0, 0, 0, 0, 0, 0, 0};
+      }
+  }
+  const Cell sxa{r[0].xd, r[0].xp, r[0].xvx, r[0].xvy}, sya{r[0].yd, r[0].yp, r[0].yvx, r[0].yvy};
+  const Cell sxb{r[1].xd, r[1].xp, r[1].xvx, r[1].xvy}, syb{r[1].yd, r[1].yp, r[1].yvx, r[1].yvy};
+  lf = interp_off(v[0], sxa, sya, fr.oxa, fr.oya);
+  rt = interp_off(v[1], sxb, syb, fr.oxb, fr.oyb);
+}
+
 // With dt > 0 the face writes its extensive update dt*area*F (the products
 // SimpleExtensiveUpdater forms, simple_extensive_updater.cpp:23-34) instead
 // of F.
@@ -535,8 +621,8 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
                                            double dt = 0.0) {
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
-  const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
-  const Cell rt = side_state(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
+  Cell lf, rt;
+  face_states(m, q, rec, m.ff[f], m.fs[f], fr, lf, rt);
   const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
   // modular_flux_calculator.cpp:56-67
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;

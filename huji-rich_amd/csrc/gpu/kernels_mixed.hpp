@@ -79,6 +79,39 @@ __device__ __forceinline__ CellF fetchf(const DMesh& m, const F4* q, int id) {
   return r;
 }
 
+// fetchf() of K point ids with the cell loads issued back to back (ghost ids
+// are resolved first, in a branch only boundary waves take).
+template <int K>
+__device__ __forceinline__ void fetchf_k(const DMesh& m, const F4* q, const int (&id)[K],
+                                         CellF (&out)[K]) {
+  int src[K];
+  bool ghost = false;
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    src[u] = id[u];
+    ghost |= id[u] >= m.N;
+  }
+  if (ghost) {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (id[u] >= m.N) src[u] = m.gsrc[id[u] - m.N];
+  }
+#pragma unroll
+  for (int u = 0; u < K; ++u) out[u] = ld_cellf(q, src[u]);
+  if (ghost && m.bc == BC_RIGID) {
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (id[u] >= m.N) {
+        const D2 n = m.gn[id[u] - m.N];
+        const float nx = static_cast<float>(n.x), ny = static_cast<float>(n.y);
+        CellF& r = out[u];
+        const float s2 = 2.f * (r.vx * nx + r.vy * ny);
+        r.vx = r.vx - nx * s2;
+        r.vy = r.vy - ny * s2;
+      }
+  }
+}
+
 __device__ __forceinline__ void add_multf(CellF& r, const CellF& o, float s) {
   r.d += o.d * s;
   r.p += o.p * s;
@@ -203,15 +236,12 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
       float2 hh[CH];
       CellF nc[CH];
       int nb[CH];
+      // slots past the last face re-read the last slot: no load behind a branch
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) {
-          nb[u] = t.enbr[eb + 64 * (s0 + u)];
-          hh[u] = t.eh[eb + 64 * (s0 + u)];
-        }
+      for (int u = 0; u < CH; ++u) nb[u] = t.enbr[eb + 64 * min(s0 + u, deg - 1)];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) nc[u] = fetchf(m, q, nb[u]);
+      for (int u = 0; u < CH; ++u) hh[u] = t.eh[eb + 64 * min(s0 + u, deg - 1)];
+      fetchf_k(m, q, nb, nc);
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         if (s0 + u >= deg) break;
@@ -239,9 +269,15 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
     const float theta = static_cast<float>(prm.theta);
     for (int s0 = 0; s0 < deg; s0 += CH) {
       float2 oo[CH];
+      CellF nc[CH];
 #pragma unroll
-      for (int u = 0; u < CH; ++u)
-        if (s0 + u < deg) oo[u] = t.eo[eb + 64 * (s0 + u)];
+      for (int u = 0; u < CH; ++u) oo[u] = t.eo[eb + 64 * min(s0 + u, deg - 1)];
+      if (shock) {
+        int nb[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) nb[u] = t.enbr[eb + 64 * min(s0 + u, deg - 1)];
+        fetchf_k(m, q, nb, nc);
+      }
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         if (s0 + u >= deg) break;
@@ -255,11 +291,10 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
           limit_fieldf(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, psi2);
           limit_fieldf(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, psi3);
         } else {
-          const CellF nc = fetchf(m, q, t.enbr[eb + 64 * (s0 + u)]);
-          shocked_fieldf(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, theta, psi0);
-          shocked_fieldf(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, theta, psi1);
-          shocked_fieldf(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, theta, psi2);
-          shocked_fieldf(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, theta, psi3);
+          shocked_fieldf(dphi.d, cv.d, cell.d, nc[u].d, maxd.d, mind.d, theta, psi0);
+          shocked_fieldf(dphi.p, cv.p, cell.p, nc[u].p, maxd.p, mind.p, theta, psi1);
+          shocked_fieldf(dphi.vx, cv.vx, cell.vx, nc[u].vx, maxd.vx, mind.vx, theta, psi2);
+          shocked_fieldf(dphi.vy, cv.vy, cell.vy, nc[u].vy, maxd.vy, mind.vy, theta, psi3);
         }
       }
     }
@@ -275,16 +310,49 @@ __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const 
   out[i] = RecM{sx.d, sx.p, sx.vx, sx.vy, sy.d, sy.p, sy.vx, sy.vy};
 }
 
-// face state of one side in FP32; (ox, oy) = centroid - CM of that side
-__device__ __forceinline__ CellF side_state_m(const DMesh& m, const F4* q, const RecM* rec, int id,
-                                              float ox, float oy) {
-  if (id >= m.N && m.bc == BC_RIGID) return fetchf(m, q, id);  // zero ghost gradient
-  const int o = id < m.N ? id : m.gorig[id - m.N];
-  const RecM r = rec[o];
-  CellF v = ld_cellf(q, o);
-  add_multf(v, CellF{r.xd, r.xp, r.xvx, r.xvy}, ox);
-  add_multf(v, CellF{r.yd, r.yp, r.yvx, r.yvy}, oy);
-  return v;
+// Face states of both sides in FP32 ((ox, oy) = centroid - CM of that side;
+// a rigid-wall ghost has zero gradient), with all loads issued together
+__device__ __forceinline__ void face_states_m(const DMesh& m, const F4* q, const RecM* rec, int a,
+                                              int b, const FaceM& fr, CellF& lt, CellF& rt) {
+  const int id[2] = {a, b};
+  int src[2] = {a, b};
+  const bool ghost = a >= m.N || b >= m.N;
+  if (ghost) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (id[u] >= m.N)
+        src[u] = m.bc == BC_PERIODIC ? m.gorig[id[u] - m.N] : m.gsrc[id[u] - m.N];
+  }
+  RecM r[2];
+  CellF v[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    r[u] = rec[src[u]];
+    v[u] = ld_cellf(q, src[u]);
+  }
+  if (ghost && m.bc == BC_RIGID) {
+    // reflected copy with zero gradient, as fetchf() forms it
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (id[u] >= m.N) {
+        const D2 n = m.gn[id[u] - m.N];
+        const float nx = static_cast<float>(n.x), ny = static_cast<float>(n.y);
+        const float s2 = 2.f * (v[u].vx * nx + v[u].vy * ny);
+        v[u].vx = v[u].vx - nx * s2;
+        v[u].vy = v[u].vy - ny * s2;
+      }
+  }
+  const bool slope[2] = {!(ghost && m.bc == BC_RIGID && a >= m.N),
+                         !(ghost && m.bc == BC_RIGID && b >= m.N)};
+  const float ox[2] = {fr.oxa, fr.oxb}, oy[2] = {fr.oya, fr.oyb};
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (slope[u]) {
+      add_multf(v[u], CellF{r[u].xd, r[u].xp, r[u].xvx, r[u].xvy}, ox[u]);
+      add_multf(v[u], CellF{r[u].yd, r[u].yp, r[u].yvx, r[u].yvy}, oy[u]);
+    }
+  lt = v[0];
+  rt = v[1];
 }
 
 // rotated HLLC in FP32 (same wave-speed estimate and star states as the FP64
@@ -340,8 +408,8 @@ __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceM fr = t.face[f];
-  const CellF lt = side_state_m(m, q, rec, m.ff[f], fr.oxa, fr.oya);
-  const CellF rt = side_state_m(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
+  CellF lt, rt;
+  face_states_m(m, q, rec, m.ff[f], m.fs[f], fr, lt, rt);
   const float wx = m.wx ? static_cast<float>(m.wx[f]) : 0.f;
   const float wy = m.wy ? static_cast<float>(m.wy[f]) : 0.f;
   const float speed = fr.nx * wx + fr.ny * wy;
