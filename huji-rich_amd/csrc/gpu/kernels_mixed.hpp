@@ -55,6 +55,12 @@ struct CellF {
   float d, p, vx, vy;
 };
 
+// The FP32 reconstruction and Riemann solver owe no bit parity to anything,
+// so their multiply-adds may fuse (FMA: fewer instructions, one rounding).
+// The FP64 update below keeps them separate, so that the two cells of a
+// face add the same rounded products and the totals are conserved.
+#pragma clang fp contract(fast)
+
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fminr(float a, float b) { return (b < a) ? b : a; }
@@ -422,6 +428,8 @@ __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4
   }
   flux[f] = o;
 }
+
+#pragma clang fp contract(off)
 
 // Neumaier-compensated FP64 sum
 struct KSum {
