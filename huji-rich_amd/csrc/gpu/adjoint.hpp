@@ -279,46 +279,57 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D
   int iamax[4] = {-1, -1, -1, -1}, iamin[4] = {-1, -1, -1, -1};
   int ibmax[4] = {-1, -1, -1, -1}, ibmin[4] = {-1, -1, -1, -1};
   Cell phy{0, 0, 0, 0}, phx{0, 0, 0, 0}, vcx{0, 0, 0, 0}, vcy{0, 0, 0, 0};
-  for (int s = 0; s < deg; ++s) {
-    const int nb = m.enbr[eb + 64 * s];
-    const Cell nc = fetch(m, q, nb);
-    if (boundary) {
-      const D2 cj = m.cm[nb];
-      const double ey = cj.y - cmi.y, ex = cj.x - cmi.x;
-      const Cell dd = sub(nc, cell);
-      phy.d += dd.d * ey;
-      phy.p += dd.p * ey;
-      phy.vx += dd.vx * ey;
-      phy.vy += dd.vy * ey;
-      phx.d += dd.d * ex;
-      phx.p += dd.p * ex;
-      phx.vx += dd.vx * ex;
-      phx.vy += dd.vy * ex;
-    } else {
-      const D2 h = m.eh[eb + 64 * s];
-      if (s == 0) {
-        vcx = Cell{cell.d * h.x, cell.p * h.x, cell.vx * h.x, cell.vy * h.x};
-        vcy = Cell{cell.d * h.y, cell.p * h.y, cell.vx * h.y, cell.vy * h.y};
+  // the gathers of CH slots at a time, issued back to back (fetch_k)
+  for (int s0 = 0; s0 < deg; s0 += CH) {
+    int nbk[CH];
+    Cell nck[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) nbk[u] = m.enbr[eb + 64 * min(s0 + u, deg - 1)];
+    fetch_k(m, q, nbk, nck);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int s = s0 + u;
+      if (s >= deg) break;
+      const int nb = nbk[u];
+      const Cell nc = nck[u];
+      if (boundary) {
+        const D2 cj = m.cm[nb];
+        const double ey = cj.y - cmi.y, ex = cj.x - cmi.x;
+        const Cell dd = sub(nc, cell);
+        phy.d += dd.d * ey;
+        phy.p += dd.p * ey;
+        phy.vx += dd.vx * ey;
+        phy.vy += dd.vy * ey;
+        phx.d += dd.d * ex;
+        phx.p += dd.p * ex;
+        phx.vx += dd.vx * ex;
+        phx.vy += dd.vy * ex;
       } else {
-        add_mult(vcx, cell, h.x);
-        add_mult(vcy, cell, h.y);
+        const D2 h = m.eh[eb + 64 * s];
+        if (s == 0) {
+          vcx = Cell{cell.d * h.x, cell.p * h.x, cell.vx * h.x, cell.vy * h.x};
+          vcy = Cell{cell.d * h.y, cell.p * h.y, cell.vx * h.y, cell.vy * h.y};
+        } else {
+          add_mult(vcx, cell, h.x);
+          add_mult(vcy, cell, h.y);
+        }
+        add_mult(vcy, nc, h.y);
+        add_mult(vcx, nc, h.x);
       }
-      add_mult(vcy, nc, h.y);
-      add_mult(vcx, nc, h.x);
-    }
-    pr = (cell.p > nc.p) ? smin(pr, nc.p / cell.p) : smin(pr, cell.p / nc.p);
-    const bool mirror = (m.einfo[eb + 64 * s] & INFO_MIRROR) != 0;
-    for (int k = 0; k < 4; ++k) {
-      const double t = fld(nc, k);
-      double* ax = k == 0 ? &amax.d : k == 1 ? &amax.p : k == 2 ? &amax.vx : &amax.vy;
-      double* an = k == 0 ? &amin.d : k == 1 ? &amin.p : k == 2 ? &amin.vx : &amin.vy;
-      if (*ax < t) { *ax = t; iamax[k] = s; }
-      if (t < *an) { *an = t; iamin[k] = s; }
-      if (!(boundary && mirror)) {
-        double* bx = k == 0 ? &bmax.d : k == 1 ? &bmax.p : k == 2 ? &bmax.vx : &bmax.vy;
-        double* bn = k == 0 ? &bmin.d : k == 1 ? &bmin.p : k == 2 ? &bmin.vx : &bmin.vy;
-        if (*bx < t) { *bx = t; ibmax[k] = s; }
-        if (t < *bn) { *bn = t; ibmin[k] = s; }
+      pr = (cell.p > nc.p) ? smin(pr, nc.p / cell.p) : smin(pr, cell.p / nc.p);
+      const bool mirror = (m.einfo[eb + 64 * s] & INFO_MIRROR) != 0;
+      for (int k = 0; k < 4; ++k) {
+        const double t = fld(nc, k);
+        double* ax = k == 0 ? &amax.d : k == 1 ? &amax.p : k == 2 ? &amax.vx : &amax.vy;
+        double* an = k == 0 ? &amin.d : k == 1 ? &amin.p : k == 2 ? &amin.vx : &amin.vy;
+        if (*ax < t) { *ax = t; iamax[k] = s; }
+        if (t < *an) { *an = t; iamin[k] = s; }
+        if (!(boundary && mirror)) {
+          double* bx = k == 0 ? &bmax.d : k == 1 ? &bmax.p : k == 2 ? &bmax.vx : &bmax.vy;
+          double* bn = k == 0 ? &bmin.d : k == 1 ? &bmin.p : k == 2 ? &bmin.vx : &bmin.vy;
+          if (*bx < t) { *bx = t; ibmax[k] = s; }
+          if (t < *bn) { *bn = t; ibmin[k] = s; }
+        }
       }
     }
   }
@@ -341,21 +352,33 @@ __global__ void __launch_bounds__(256) k_adj_slopes(DMesh m, AdjMesh am, const D
     const double cs = dp2c(prm.gamma, cell.d, cell.p);
     shock = ((sx.vx + sy.vy) * g.width < (-prm.delta_v) * cs) || (pr < prm.delta_p);
     const Cell maxd = sub(shock ? amax : bmax, cell), mind = sub(shock ? amin : bmin, cell);
-    for (int s = 0; s < deg; ++s) {
-      const D2 o = m.eo[eb + 64 * s];
-      const Cell cv = interp_off(cell, sx, sy, o.x, o.y);
-      const Cell dphi = sub(cv, cell);
-      if (!shock) {
-        lim_field(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, s, lp[0]);
-        lim_field(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, s, lp[1]);
-        lim_field(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, s, lp[2]);
-        lim_field(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, s, lp[3]);
-      } else {
-        const Cell nc = fetch(m, q, m.enbr[eb + 64 * s]);
-        shocked_lim_field(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, prm.theta, s, lp[0]);
-        shocked_lim_field(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, prm.theta, s, lp[1]);
-        shocked_lim_field(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, prm.theta, s, lp[2]);
-        shocked_lim_field(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, prm.theta, s, lp[3]);
+    for (int s0 = 0; s0 < deg; s0 += CH) {
+      Cell nck[CH];
+      if (shock) {
+        int nbk[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) nbk[u] = m.enbr[eb + 64 * min(s0 + u, deg - 1)];
+        fetch_k(m, q, nbk, nck);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int s = s0 + u;
+        if (s >= deg) break;
+        const D2 o = m.eo[eb + 64 * s];
+        const Cell cv = interp_off(cell, sx, sy, o.x, o.y);
+        const Cell dphi = sub(cv, cell);
+        if (!shock) {
+          lim_field(dphi.d, cv.d, cell.d, maxd.d, mind.d, false, s, lp[0]);
+          lim_field(dphi.p, cv.p, cell.p, maxd.p, mind.p, false, s, lp[1]);
+          lim_field(dphi.vx, cv.vx, cell.vx, maxd.vx, mind.vx, true, s, lp[2]);
+          lim_field(dphi.vy, cv.vy, cell.vy, maxd.vy, mind.vy, true, s, lp[3]);
+        } else {
+          const Cell nc = nck[u];
+          shocked_lim_field(dphi.d, cv.d, cell.d, nc.d, maxd.d, mind.d, prm.theta, s, lp[0]);
+          shocked_lim_field(dphi.p, cv.p, cell.p, nc.p, maxd.p, mind.p, prm.theta, s, lp[1]);
+          shocked_lim_field(dphi.vx, cv.vx, cell.vx, nc.vx, maxd.vx, mind.vx, prm.theta, s, lp[2]);
+          shocked_lim_field(dphi.vy, cv.vy, cell.vy, nc.vy, maxd.vy, mind.vy, prm.theta, s, lp[3]);
+        }
       }
     }
   }
