@@ -371,21 +371,26 @@ __device__ __forceinline__ void minmax(Cell& mx, Cell& mn, const Cell& t) {
 constexpr int CH = 4;
 constexpr int CH_UPD = 8;
 
-// ELL copies of the per-incidence geometry (one thread per cell over its
-// faces): the same values the slope and update loops would read through the
-// face index, so the results do not change.
-__global__ void __launch_bounds__(256) k_geo_ell(DMesh m, D2* __restrict__ eh, D2* __restrict__ eo,
-                                                 double* __restrict__ earea) {
+// The ELL arrays (one thread per cell over its faces): the incidences, and
+// copies of the per-incidence geometry -- the same values the slope and
+// update loops would read through the face index, so the results do not
+// change.
+__global__ void __launch_bounds__(256) k_geo_ell(DMesh m, int* __restrict__ enbr,
+                                                 unsigned* __restrict__ einfo, D2* __restrict__ eh,
+                                                 D2* __restrict__ eo, double* __restrict__ earea) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const D2 ci = m.cm[i];
   const int base = ell_base(m, i);
   for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
-    const unsigned info = m.inc[k].info;
+    const Inc ic = m.inc[k];
+    const unsigned info = ic.info;
+    const int at = base + 64 * (k - m.off[i]);
+    enbr[at] = ic.nbr;
+    einfo[at] = info;
     const int f = static_cast<int>(info & INFO_FACE);
     const FaceRec r = m.frec[f];
     const bool second = (info & INFO_SECOND) != 0;
-    const int at = base + 64 * (k - m.off[i]);
     eh[at] = D2{second ? -r.hx : r.hx, second ? -r.hy : r.hy};
     eo[at] = D2{r.cx - ci.x, r.cy - ci.y};
     earea[at] = m.area[f];

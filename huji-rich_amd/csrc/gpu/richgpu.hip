@@ -147,7 +147,6 @@ struct rg_ctx {
     std::vector<int> ff, fs;     // [F]
     std::vector<int> gsrc;       // [G]
     std::vector<int> wofs;       // [waves+1]
-    std::vector<int> epos;       // [2F] ELL entry of each face's first and second side
   } adj_src;
 
   // moving mesh: point velocities kept across rg_replace_mesh, and where
@@ -570,6 +569,16 @@ hipError_t build_adjoint_tables(rg_ctx* c) {
     return static_cast<size_t>(a.wofs[i / 64] + i % 64) +
            64 * static_cast<size_t>(k - a.csr_off[i]);
   };
+  // ELL entry of each face on its first [2f] and second [2f + 1] side
+  std::vector<int> epos(2 * static_cast<size_t>(F), -1);
+  rmp::parallel_for(N, [&](int64_t ii) {  // each (face, side) has one cell
+    const int i = static_cast<int>(ii);
+    for (int k = a.csr_off[i]; k < a.csr_off[i + 1]; ++k) {
+      const unsigned w = a.inc[k].info;
+      epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 1 : 0)] =
+          static_cast<int>(ell_at(i, k));
+    }
+  });
   std::vector<int> rev(ne, -1);
   std::vector<int> gf_off(static_cast<size_t>(N) + 1, 0), ge_off(static_cast<size_t>(N) + 1, 0);
   for (int f = 0; f < F; ++f) {
@@ -584,7 +593,7 @@ hipError_t build_adjoint_tables(rg_ctx* c) {
         continue;
       }
       const unsigned w = a.inc[k].info;
-      rev[a.epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 0 : 1)]] =
+      rev[epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 0 : 1)]] =
           static_cast<int>(ell_at(i, k));
     }
   for (int i = 0; i < N; ++i) {
@@ -755,15 +764,32 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       !m->pt_x || !m->pt_y || !m->cm_x || !m->cm_y || !m->volume || !m->tess_volume ||
       !m->csr_off || !m->csr_face || (G > 0 && !m->ghost_orig))
     return RG_E_ARG;
-  // --- validate topology on the host
+  // --- validate topology on the host threads
   if (m->csr_off[0] != 0) return RG_E_ARG;
   for (int i = 0; i < N; ++i)
     if (m->csr_off[i + 1] < m->csr_off[i]) return RG_E_ARG;
   const int nnz = m->csr_off[N];
-  for (int f = 0; f < F; ++f) {
-    const int a = m->face_first[f], b = m->face_second[f];
-    if (a < 0 || b < 0 || a >= N + G || b >= N + G || a == b) return RG_E_ARG;
-    if (a >= N && b >= N) return RG_E_ARG;
+  int64_t faces_rr = 0;  // faces between two real cells
+  {
+    const int nc = rmp::chunks(F);
+    std::vector<char> bad(static_cast<size_t>(nc), 0);
+    std::vector<int64_t> rr(static_cast<size_t>(nc), 0);
+    rmp::for_chunks(F, nc, [&](int ch, int64_t f0, int64_t f1) {
+      int64_t n = 0;  // a local count: per-chunk counters share cache lines
+      for (int64_t f = f0; f < f1; ++f) {
+        const int a = m->face_first[f], b = m->face_second[f];
+        if (a < 0 || b < 0 || a >= N + G || b >= N + G || a == b || (a >= N && b >= N)) {
+          bad[ch] = 1;
+          return;
+        }
+        if (a < N && b < N) ++n;
+      }
+      rr[ch] = n;
+    });
+    for (int ch = 0; ch < nc; ++ch) {
+      if (bad[ch]) return RG_E_ARG;
+      faces_rr += rr[ch];
+    }
   }
   for (int g = 0; g < G; ++g)
     if (m->ghost_orig[g] < 0 || m->ghost_orig[g] >= N) return RG_E_ARG;
@@ -773,7 +799,9 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     // each cell's incidences on host threads; any bad entry fails the mesh
     const int nc = rmp::chunks(N);
     std::vector<char> bad(static_cast<size_t>(nc), 0);
+    std::vector<int64_t> inc_rr(static_cast<size_t>(nc), 0);
     rmp::for_chunks(N, nc, [&](int ch, int64_t i0, int64_t i1) {
+      int64_t n_rr = 0;
       for (int i = static_cast<int>(i0); i < static_cast<int>(i1); ++i) {
         int prev = -1;
         for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
@@ -792,11 +820,21 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
           if (b == i) w |= INFO_SECOND;
           if (orig(a) == orig(b)) w |= INFO_MIRROR;
           inc[k] = Inc{(a == i) ? b : a, w};
+          if (a < N && b < N) ++n_rr;
         }
       }
+      inc_rr[ch] = n_rr;
     });
-    for (char v : bad)
-      if (v) return RG_E_ARG;
+    int64_t listed_rr = 0;
+    for (int ch = 0; ch < nc; ++ch) {
+      if (bad[ch]) return RG_E_ARG;
+      listed_rr += inc_rr[ch];
+    }
+    // Every face between two real cells must be in both cells' lists.  A
+    // cell lists a face at most once (ascending order) and only as one of
+    // its sides (checked above), so that holds exactly when such faces are
+    // listed twice as often as there are of them.
+    if (listed_rr != 2 * faces_rr) return RG_E_ARG;
   }
   // Ghost sources: RigidWallGenerator copies the real cell of the ghost's
   // first boundary face (its flat_map insert keeps the first) and reverses
@@ -829,7 +867,8 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     }
   }
   std::vector<D2> cm(static_cast<size_t>(N) + G);
-  for (int i = 0; i < N + G; ++i) cm[i] = D2{m->cm_x[i], m->cm_y[i]};
+  rmp::parallel_for(static_cast<int64_t>(N) + G,
+                    [&](int64_t i) { cm[i] = D2{m->cm_x[i], m->cm_y[i]}; });
   // --- upload
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drain_pool(c);
@@ -880,47 +919,17 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     return static_cast<size_t>(wofs[i / 64] + i % 64) +
            64 * static_cast<size_t>(k - m->csr_off[i]);
   };
-  std::vector<int> enbr(ne, 0);
-  std::vector<unsigned> einfo(ne, 0u);
-  rmp::parallel_for(N, [&](int64_t ii) {
-    const int i = static_cast<int>(ii);
-    for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-      enbr[ell_at(i, k)] = inc[k].nbr;
-      einfo[ell_at(i, k)] = inc[k].info;
-    }
-  });
-  // ELL entry of each face on its first [2f] and second [2f + 1] side; every
-  // face between two real cells must be in both cells' lists
-  std::vector<int> epos(2 * static_cast<size_t>(F), -1);
-  rmp::parallel_for(N, [&](int64_t ii) {  // each (face, side) has one cell
-    const int i = static_cast<int>(ii);
-    for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) {
-      const unsigned w = inc[k].info;
-      epos[2 * static_cast<size_t>(w & INFO_FACE) + ((w & INFO_SECOND) ? 1 : 0)] =
-          static_cast<int>(ell_at(i, k));
-    }
-  });
-  {
-    const int nc = rmp::chunks(nnz);
-    std::vector<char> bad(static_cast<size_t>(nc), 0);
-    rmp::for_chunks(nnz, nc, [&](int ch, int64_t k0, int64_t k1) {
-      for (int64_t k = k0; k < k1; ++k)
-        if (inc[k].nbr < N && epos[2 * static_cast<size_t>(inc[k].info & INFO_FACE) +
-                                   ((inc[k].info & INFO_SECOND) ? 0 : 1)] < 0) {
-          bad[ch] = 1;  // face missing from the neighbour's list
-          return;
-        }
-    });
-    for (char v : bad)
-      if (v) return RG_E_ARG;
-  }
+  // the ELL arrays are filled on the device from the incidences (k_geo_ell);
+  // padding entries stay zero
   int *wofs_d, *enbr_d;
   unsigned* einfo_d;
   D2 *eh, *eo;
   double* earea;
   HIPCHK(c, upload(c, &wofs_d, wofs.data(), wofs.size()));
-  HIPCHK(c, upload(c, &enbr_d, enbr.data(), ne));
-  HIPCHK(c, upload(c, &einfo_d, einfo.data(), ne));
+  HIPCHK(c, dalloc(c, &enbr_d, ne));
+  HIPCHK(c, dalloc(c, &einfo_d, ne));
+  HIPCHK(c, hipMemsetAsync(enbr_d, 0, sizeof(int) * ne, c->stream));
+  HIPCHK(c, hipMemsetAsync(einfo_d, 0, sizeof(unsigned) * ne, c->stream));
   HIPCHK(c, dalloc(c, &eh, ne));
   HIPCHK(c, dalloc(c, &eo, ne));
   HIPCHK(c, dalloc(c, &earea, ne));
@@ -968,7 +977,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     k_geo_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeo, width);
   }));
   HIPCHK(c, launch(c, KID_GEO, [&] {
-    k_geo_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, eh, eo, earea);
+    k_geo_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, enbr_d, einfo_d, eh, eo, earea);
   }));
   // state
   HIPCHK(c, dalloc(c, &c->prim, N));
@@ -1065,7 +1074,6 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   c->adj_src.fs.assign(m->face_second, m->face_second + F);
   c->adj_src.gsrc.assign(gsrc.begin(), gsrc.begin() + G);
   c->adj_src.wofs = std::move(wofs);
-  c->adj_src.epos = std::move(epos);
   c->has_mesh = true;
   drain_pool(c);
   return RG_OK;
