@@ -108,7 +108,11 @@ void rg_destroy(rg_ctx* ctx);
 int rg_set_stream(rg_ctx* ctx, void* hip_stream);
 
 /* mesh and state (host arrays) ------------------------------------- */
-/* Upload a mesh (hdsim holds the Tessellation, hdsim2d.hpp:54). */
+/* Upload a mesh (hdsim holds the Tessellation, hdsim2d.hpp:54).  RG_E_ARG,
+ * with nothing uploaded, unless: side ids are in [0, N+G), differ, and at
+ * least one is a real cell; csr_off starts at 0 and never decreases; each
+ * cell lists faces it is a side of, in ascending order; every face between
+ * two real cells is listed by both; ghost_orig is in [0, N). */
 int rg_set_mesh(rg_ctx* ctx, const rg_mesh* mesh);
 /* Face velocities [F] as hdsim passes edge_velocities; NULL,NULL = all zero
  * (StationaryBox/PeriodicEdgeVelocities on a static mesh). */
