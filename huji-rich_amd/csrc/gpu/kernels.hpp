@@ -2,26 +2,30 @@
 //
 // Per mesh (static geometry, computed once by k_geo_*):
 //   per face  FaceRec   {h = r_ij/2 seen from the first side, centroid}
-//             FaceFrame {centroid, rotated-frame normal n, |n|, parallel p, |p|}
+//             FaceFrame {centroid minus each side's CM, rotated-frame
+//                        normal n, face direction p}
 //             area
-//   per cell  CellGeo   {CM, width, kind, inverse gradient matrix} -- the
-//             matrix of calc_naive_slope (or the least-squares sums of
-//             GetBoundarySlope) depends on geometry only
+//   per cell  CellGeo   {inverse gradient matrix, width, kind, face count}
+//             -- the matrix of calc_naive_slope (or the least-squares sums
+//             of GetBoundarySlope) depends on geometry only
+//   per incidence (wavefront-sliced ELL) neighbour, face word, h, centroid
+//             minus the cell's CM, area
 // Per stage:
 //   k_slopes  one thread per cell: the value-dependent half of
 //             LinearGaussImproved (gradient right-hand side in face order,
-//             shock detector, both limiters); writes a 128-byte record
-//             {cell, x-slope, y-slope, CM} read by the face kernel
+//             shock detector, both limiters); writes a 64-byte record of
+//             the limited x- and y-slopes read by the face kernel
 //   k_flux2   one thread per face: face states, rotated HLLC, rotate back
 //   k_flux1   one thread per face: first-order SimpleFluxCalculator
 //   k_update  one thread per cell: ascending-face segmented sum of
 //             -/+ dt*A*F, ZeroForce, Heun average, primitive recovery and
-//             the CFL term; block minima are combined by the last block
+//             the CFL term with per-block minima; k_reduce_min combines
+//             them
 // Precomputing a quantity once evaluates the same IEEE operations the
 // reference evaluates on every call, so results stay bit-identical.
-// Each cell gathers its faces through a CSR in ascending face order, the
-// order in which the reference scatters face contributions: sums are
-// deterministic and equal to the reference's, with no atomics.
+// Each cell gathers its faces in ascending face order, the order in which
+// the reference scatters face contributions: sums are deterministic and
+// equal to the reference's, with no atomics.
 #pragma once
 
 #include "physics.hpp"
