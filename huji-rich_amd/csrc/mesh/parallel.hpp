@@ -96,7 +96,8 @@ inline Pool& pool() {
 
 // fn(chunk, begin, end) for `nc` contiguous chunks of [0, n); chunk 0 runs
 // on the calling thread, the others on the pool.  Called from inside a
-// chunk (a pool worker), it runs its chunks in order on that thread.
+// chunk (a pool worker), or with one thread, it runs its chunks in order on
+// the calling thread.
 template <class Fn>
 void for_chunks(int64_t n, int nc, Fn&& fn) {
   auto range = [&](int k, int64_t& b, int64_t& e) {
@@ -104,7 +105,7 @@ void for_chunks(int64_t n, int nc, Fn&& fn) {
     e = n * (k + 1) / nc;
   };
   int64_t b, e;
-  if (nc <= 1 || detail::in_worker()) {
+  if (nc <= 1 || threads() <= 1 || detail::in_worker()) {  // no pool workers to wait for
     for (int k = 0; k < std::max(nc, 1); ++k) {
       range(k, b, e);
       fn(k, b, e);
