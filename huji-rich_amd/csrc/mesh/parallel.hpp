@@ -105,8 +105,12 @@ void for_chunks(int64_t n, int nc, Fn&& fn) {
     e = n * (k + 1) / nc;
   };
   int64_t b, e;
-  if (nc <= 1 || threads() <= 1 || detail::in_worker()) {  // no pool workers to wait for
-    for (int k = 0; k < std::max(nc, 1); ++k) {
+  if (nc <= 1) {
+    fn(0, int64_t{0}, n);
+    return;
+  }
+  if (threads() <= 1 || detail::in_worker()) {  // no pool workers to wait for
+    for (int k = 0; k < nc; ++k) {
       range(k, b, e);
       fn(k, b, e);
     }
