@@ -100,3 +100,28 @@ def test_set_mesh_rejects_bad_topology(name):
         with pytest.raises(RichGPUError) as e:
             RichGPU(bad, order=2)
         assert e.value.status == RG_E_ARG, what
+
+
+def test_rejected_swap_leaves_the_context_intact():
+    """rg_replace_mesh with a corrupted mesh fails before touching the
+    device: the context steps on exactly as one that never saw it."""
+    m = dict(cases.small_meshes())["rigid"]
+    cells = cases.rough_cells(m, seed=5)
+    cells[0] = np.abs(cells[0]) + 0.5
+    cells[1] = np.abs(cells[1]) + 0.5
+    runs = []
+    for swap in (False, True):
+        g = RichGPU(_copy(m), order=2)
+        g.set_cells(cells)
+        g.init_extensives()
+        g.advance(3, integrator=2)
+        if swap:
+            for what, bad in _corruptions(m):
+                with pytest.raises(RichGPUError) as e:
+                    g.replace_mesh(bad)
+                assert e.value.status == RG_E_ARG, what
+        g.advance(3, integrator=2)
+        runs.append((g.get_cells(), g.get_extensives()))
+        g.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
