@@ -173,14 +173,29 @@ __device__ __forceinline__ void cell_state_adj(const DMesh& m, const AdjMesh& am
                                                const D4* rb, int i, int deg, int eb, Cell& v,
                                                Cell& sx, Cell& sy) {
   v = sx = sy = Cell{0, 0, 0, 0};
-  for (int s = 0; s < deg; ++s) {
-    const unsigned info = m.einfo[eb + 64 * s];
-    const int f = static_cast<int>(info & INFO_FACE);
-    const Cell g = d4_cell((info & INFO_SECOND) ? rb[f] : lb[f]);
-    const D2 o = m.eo[eb + 64 * s];
-    acc(v, g);
-    acc_scaled(sx, g, o.x);
-    acc_scaled(sy, g, o.y);
+  // CH incidences at a time: their face words and offsets, then their face
+  // states' adjoints, each batch of loads issued together; sums in slot order
+  for (int s0 = 0; s0 < deg; s0 += CH) {
+    unsigned info[CH];
+    D2 o[CH];
+    D4 gv[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) info[u] = m.einfo[eb + 64 * min(s0 + u, deg - 1)];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) o[u] = m.eo[eb + 64 * min(s0 + u, deg - 1)];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int f = static_cast<int>(info[u] & INFO_FACE);
+      gv[u] = (info[u] & INFO_SECOND) ? rb[f] : lb[f];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (s0 + u >= deg) break;
+      const Cell g = d4_cell(gv[u]);
+      acc(v, g);
+      acc_scaled(sx, g, o[u].x);
+      acc_scaled(sy, g, o[u].y);
+    }
   }
   for (int k = am.gf_off[i]; k < am.gf_off[i + 1]; ++k) {
     const unsigned w = am.gf[k];
@@ -477,9 +492,20 @@ __global__ void __launch_bounds__(256) k_adj_gather(DMesh m, AdjMesh am,
   if (i < m.N) {
     Cell v = d4_cell(gps[i]);
     const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
-    for (int s = 0; s < deg; ++s) {
-      const int r = am.rev[eb + 64 * s];
-      if (r >= 0) acc(v, d4_cell(emit[r]));
+    // CH incidences at a time: the ELL entries of their twins, then the
+    // shares emitted there (entry 0 stands in for a ghost side, unused)
+    for (int s0 = 0; s0 < deg; s0 += CH) {
+      int r[CH];
+      D4 e[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) r[u] = am.rev[eb + 64 * min(s0 + u, deg - 1)];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) e[u] = emit[r[u] >= 0 ? r[u] : 0];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        if (s0 + u >= deg) break;
+        if (r[u] >= 0) acc(v, d4_cell(e[u]));
+      }
     }
     for (int k = am.ge_off[i]; k < am.ge_off[i + 1]; ++k) {
       const int idx = am.ge[k];
