@@ -108,8 +108,20 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
   auto before = [](const EdgeRef& a, const EdgeRef& b) {
     return a.tlo != b.tlo ? a.tlo < b.tlo : a.k < b.k;
   };
-  if (!std::is_sorted(edges.begin(), edges.end(), before))
-    std::sort(edges.begin(), edges.end(), before);
+  {
+    const int64_t ne = static_cast<int64_t>(edges.size());
+    const int nc = rmp::chunks(ne);
+    std::vector<char> sorted(static_cast<size_t>(nc), 1);
+    rmp::for_chunks(ne, nc, [&](int c, int64_t e0, int64_t e1) {
+      for (int64_t i = std::max<int64_t>(e0, 1); i < e1; ++i)
+        if (before(edges[i], edges[i - 1])) {
+          sorted[c] = 0;
+          break;
+        }
+    });
+    if (std::find(sorted.begin(), sorted.end(), 0) != sorted.end())
+      std::sort(edges.begin(), edges.end(), before);
+  }
   // 6. faces, dropping near-zero-length ones (VoronoiMesh.cpp:287-288)
   uint64_t fs = static_cast<uint64_t>(flip_seed) * 0x9E3779B97F4A7C15ull + 12345u;
   auto long_enough = [&](const EdgeRef& e) {
@@ -123,9 +135,13 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
     const int64_t ne = static_cast<int64_t>(edges.size());
     const int nc = rmp::chunks(ne);
     std::vector<int64_t> cnt(static_cast<size_t>(nc) + 1, 0);
+    std::vector<char> kept(static_cast<size_t>(ne));
     rmp::for_chunks(ne, nc, [&](int c, int64_t e0, int64_t e1) {
       int64_t k = 0;
-      for (int64_t i = e0; i < e1; ++i) k += long_enough(edges[i]) ? 1 : 0;
+      for (int64_t i = e0; i < e1; ++i) {
+        kept[i] = long_enough(edges[i]) ? 1 : 0;
+        k += kept[i];
+      }
       cnt[c + 1] = k;
     });
     for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
@@ -139,8 +155,8 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
     rmp::for_chunks(ne, nc, [&](int c, int64_t e0, int64_t e1) {
       int64_t f = cnt[c];
       for (int64_t i = e0; i < e1; ++i) {
+        if (!kept[i]) continue;
         const EdgeRef& e = edges[i];
-        if (!long_enough(e)) continue;
         const Tri &tl = tris[e.tlo], &th = tris[e.thi];
         m->ff[f] = e.first;
         m->fs[f] = e.second;
@@ -175,19 +191,25 @@ void finish_mesh(rm_mesh* m, const std::vector<Tri>& tris, std::vector<EdgeRef>&
   }
   m->F = static_cast<int32_t>(m->ff.size());
   // CSR: each cell's faces in ascending face order
+  // (counts and slots taken with atomic adds; each cell's list is then
+  // sorted, so the result does not depend on timing)
   m->off.assign(static_cast<size_t>(N) + 1, 0);
-  for (int32_t f = 0; f < m->F; ++f) {
-    if (m->ff[f] < N) ++m->off[m->ff[f] + 1];
-    if (m->fs[f] < N) ++m->off[m->fs[f] + 1];
-  }
+  rmp::parallel_for(m->F, [&](int64_t f) {
+    if (m->ff[f] < N) __atomic_fetch_add(&m->off[m->ff[f] + 1], 1, __ATOMIC_RELAXED);
+    if (m->fs[f] < N) __atomic_fetch_add(&m->off[m->fs[f] + 1], 1, __ATOMIC_RELAXED);
+  });
   for (int32_t c = 0; c < N; ++c) m->off[c + 1] += m->off[c];
   m->csr.resize(static_cast<size_t>(m->off[N]));
   {
     std::vector<int32_t> fill(m->off.begin(), m->off.end() - 1);
-    for (int32_t f = 0; f < m->F; ++f) {
-      if (m->ff[f] < N) m->csr[fill[m->ff[f]]++] = f;
-      if (m->fs[f] < N) m->csr[fill[m->fs[f]]++] = f;
-    }
+    rmp::parallel_for(m->F, [&](int64_t f) {
+      const int32_t fi = static_cast<int32_t>(f);
+      if (m->ff[f] < N) m->csr[__atomic_fetch_add(&fill[m->ff[f]], 1, __ATOMIC_RELAXED)] = fi;
+      if (m->fs[f] < N) m->csr[__atomic_fetch_add(&fill[m->fs[f]], 1, __ATOMIC_RELAXED)] = fi;
+    });
+    rmp::parallel_for(N, [&](int64_t c) {
+      std::sort(m->csr.begin() + m->off[c], m->csr.begin() + m->off[c + 1]);
+    });
   }
 
   // 7. cell geometry, in each cell's face order
@@ -514,23 +536,31 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
   // triangles numbered by their point ids (smallest first, then the next
   // two counter-clockwise), so they, and the faces after them, follow the
   // cell order and depend only on the set of triangles
+  // bucketed by the smallest id: counts and slots taken with atomic adds,
+  // so a bucket's initial order varies with timing; the sort below orders
+  // it completely
   std::vector<int32_t> cnt(static_cast<size_t>(NP) + 1, 0);
-  for (int32_t t = 0; t < n_tri; ++t) {
+  int bad_id = 0;
+  rmp::parallel_for(n_tri, [&](int64_t t) {
     const int32_t* q = tri + 3 * static_cast<size_t>(t);
     for (int k = 0; k < 3; ++k)
       if (q[k] < 0 || q[k] >= NP) {
-        delete m;
-        return nullptr;
+        __atomic_store_n(&bad_id, 1, __ATOMIC_RELAXED);
+        return;
       }
-    ++cnt[std::min(q[0], std::min(q[1], q[2])) + 1];
+    __atomic_fetch_add(&cnt[std::min(q[0], std::min(q[1], q[2])) + 1], 1, __ATOMIC_RELAXED);
+  });
+  if (bad_id) {
+    delete m;
+    return nullptr;
   }
   for (int32_t i = 0; i < NP; ++i) cnt[i + 1] += cnt[i];
   std::vector<std::pair<int32_t, int32_t>> order(static_cast<size_t>(n_tri));
-  for (int32_t t = 0; t < n_tri; ++t) {
+  rmp::parallel_for(n_tri, [&](int64_t t) {
     const int32_t* q = tri + 3 * static_cast<size_t>(t);
     const int32_t lo = std::min(q[0], std::min(q[1], q[2]));
-    order[cnt[lo]++] = {lo, t};
-  }
+    order[__atomic_fetch_add(&cnt[lo], 1, __ATOMIC_RELAXED)] = {lo, static_cast<int32_t>(t)};
+  });
   {
     // within one smallest id: by the ids that follow it (orientation as
     // given, which is counter-clockwise where `half` is)
@@ -549,14 +579,21 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
         while (e < no && order[e].first == order[b].first) ++e;
         std::sort(order.begin() + b, order.begin() + e,
                   [&](const std::pair<int32_t, int32_t>& u, const std::pair<int32_t, int32_t>& v) {
-                    return next2(u.second) < next2(v.second);
+                    const auto a = next2(u.second), b = next2(v.second);
+                    return a != b ? a < b : u.second < v.second;  // ties: repeated triangles
                   });
         b = e;
       }
     });
   }
   std::vector<Tri> tris(order.size());
-  std::vector<int8_t> rot(half ? order.size() : 0);
+  // with `half`: each triangle's position in the order above, and hr, the
+  // position of the triangle across each edge of the reordered triangle
+  // (-1 on the hull), read in order by the edge pass below
+  std::vector<int32_t> rank(half ? static_cast<size_t>(n_tri) : 0);
+  std::vector<int32_t> hr(half ? 3 * order.size() : 0);
+  if (half)
+    rmp::parallel_for(n_tri, [&](int64_t r) { rank[order[r].second] = static_cast<int32_t>(r); });
   rmp::parallel_for(static_cast<int64_t>(order.size()), [&](int64_t r) {
     const int32_t* q = tri + 3 * static_cast<size_t>(order[r].second);
     int32_t a = q[0], b = q[1], c = q[2];
@@ -580,7 +617,13 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
       b = a;
       a = t0;
     }
-    if (half) rot[r] = static_cast<int8_t>(k0);
+    if (half) {
+      const int32_t* hq = half + 3 * static_cast<size_t>(order[r].second);
+      for (int k = 0; k < 3; ++k) {
+        const int32_t h = hq[(k + k0) % 3];
+        hr[3 * static_cast<size_t>(r) + k] = h >= 0 ? rank[h / 3] : -1;
+      }
+    }
     Tri t;
     t.p[0] = a;
     t.p[1] = b;
@@ -597,31 +640,36 @@ rm_mesh* points_mesh(int32_t N, int32_t G, const double* x, const double* y,
   // shared edges: the first triangle (in the order above) to list an edge
   // owns it; the second one closes it
   std::vector<EdgeRef> edges;
-  edges.reserve(tris.size() * 3 / 2);
   if (half) {
-    std::vector<int32_t> rank(static_cast<size_t>(n_tri));
-    rmp::parallel_for(n_tri, [&](int64_t r) { rank[order[r].second] = static_cast<int32_t>(r); });
-    // per chunk of triangles, joined in chunk order (the sequential order)
+    // counted per chunk of triangles, then written in place (the
+    // sequential order)
+    auto owns = [&](int32_t r, int k) {
+      const int32_t r2 = hr[3 * static_cast<size_t>(r) + k];
+      if (r2 <= r) return false;  // hull edge (-1), or owned by an earlier triangle
+      return tris[r].p[k] < N || tris[r].p[(k + 1) % 3] < N;  // two ghosts: not a face
+    };
     const int nc = rmp::chunks(n_tri);
-    std::vector<std::vector<EdgeRef>> part(static_cast<size_t>(nc));
+    std::vector<int64_t> ecnt(static_cast<size_t>(nc) + 1, 0);
     rmp::for_chunks(n_tri, nc, [&](int c, int64_t r0, int64_t r1) {
-      part[c].reserve(static_cast<size_t>((r1 - r0) * 3 / 2 + 16));
+      int64_t k = 0;
+      for (int64_t r = r0; r < r1; ++r)
+        for (int j = 0; j < 3; ++j) k += owns(static_cast<int32_t>(r), j) ? 1 : 0;
+      ecnt[c + 1] = k;
+    });
+    for (int c = 0; c < nc; ++c) ecnt[c + 1] += ecnt[c];
+    edges.resize(static_cast<size_t>(ecnt[nc]));
+    rmp::for_chunks(n_tri, nc, [&](int c, int64_t r0, int64_t r1) {
+      int64_t o = ecnt[c];
       for (int64_t rr = r0; rr < r1; ++rr) {
         const int32_t r = static_cast<int32_t>(rr);
-        const int32_t t0 = order[r].second;
-        for (int k = 0; k < 3; ++k) {
-          const int32_t h = half[3 * static_cast<size_t>(t0) + (k + rot[r]) % 3];
-          if (h < 0) continue;
-          const int32_t r2 = rank[h / 3];
-          if (r2 <= r) continue;
-          const int32_t a = tris[r].p[k], b = tris[r].p[(k + 1) % 3];
-          if (a >= N && b >= N) continue;  // both ghosts: not a face of the mesh
-          part[c].push_back(EdgeRef{r, r2, a, b, k});
-        }
+        for (int k = 0; k < 3; ++k)
+          if (owns(r, k))
+            edges[o++] = EdgeRef{r, hr[3 * static_cast<size_t>(r) + k], tris[r].p[k],
+                                 tris[r].p[(k + 1) % 3], k};
       }
     });
-    for (auto& v : part) edges.insert(edges.end(), v.begin(), v.end());
   } else {
+    edges.reserve(tris.size() * 3 / 2);
     std::unordered_map<uint64_t, std::pair<int32_t, int32_t>> open;  // key -> (triangle, slot)
     open.reserve(tris.size() * 2);
     for (int32_t t = 0; t < static_cast<int32_t>(tris.size()); ++t)
@@ -701,17 +749,37 @@ rm_mesh* mesh_of_triangulation(int32_t N, const std::vector<double>& X,
                                double y1) {
   const int32_t T = static_cast<int32_t>(tri.size() / 3);
   const int32_t G0 = static_cast<int32_t>(go.size());
+  // kept triangles (those with a real point) numbered in order: counted
+  // per chunk, then numbered in place; the ghosts they use are marked
   std::vector<int32_t> keep(static_cast<size_t>(T), -1), gid(static_cast<size_t>(G0), -1);
-  int32_t nt = 0;
-  for (int32_t t = 0; t < T; ++t) {
+  auto kept = [&](int64_t t) {
     const int32_t* q = &tri[3 * static_cast<size_t>(t)];
-    if (q[0] >= N && q[1] >= N && q[2] >= N) continue;
-    keep[t] = nt++;
-    for (int k = 0; k < 3; ++k) {
-      if (q[k] >= N + G0) return nullptr;  // a real point on the hull of its reflections
-      if (q[k] >= N) gid[q[k] - N] = 0;
+    return q[0] < N || q[1] < N || q[2] < N;
+  };
+  const int nc = rmp::chunks(T);
+  std::vector<int32_t> kcnt(static_cast<size_t>(nc) + 1, 0);
+  int frame_pt = 0;
+  rmp::for_chunks(T, nc, [&](int c, int64_t t0, int64_t t1) {
+    int32_t n = 0;
+    for (int64_t t = t0; t < t1; ++t) {
+      if (!kept(t)) continue;
+      ++n;
+      const int32_t* q = &tri[3 * static_cast<size_t>(t)];
+      for (int k = 0; k < 3; ++k) {
+        if (q[k] >= N + G0) __atomic_store_n(&frame_pt, 1, __ATOMIC_RELAXED);
+        else if (q[k] >= N) __atomic_store_n(&gid[q[k] - N], 0, __ATOMIC_RELAXED);
+      }
     }
-  }
+    kcnt[c + 1] = n;
+  });
+  if (frame_pt) return nullptr;  // a real point on the hull of its reflections
+  for (int c = 0; c < nc; ++c) kcnt[c + 1] += kcnt[c];
+  const int32_t nt = kcnt[nc];
+  rmp::for_chunks(T, nc, [&](int c, int64_t t0, int64_t t1) {
+    int32_t n = kcnt[c];
+    for (int64_t t = t0; t < t1; ++t)
+      if (kept(t)) keep[t] = n++;
+  });
   std::vector<double> px(X.begin(), X.begin() + N), py(Y.begin(), Y.begin() + N);
   std::vector<int32_t> gorig, gwall;
   int32_t G = 0;
