@@ -399,13 +399,23 @@ int rmd::repair(int32_t n, const double* x, const double* y, std::vector<int32_t
       if (v) return -6;
   }
   // the boundary (edges with no twin, interior on their left) still convex
+  // (the few edges with no twin are found in parallel and taken in edge order)
   std::vector<int32_t> hn(static_cast<size_t>(n), -1);
-  for (int32_t e = 0; e < 3 * T; ++e)
-    if (half[e] < 0) {
-      const int32_t a = tri[e], b = tri[e - e % 3 + (e % 3 + 1) % 3];
-      if (hn[a] >= 0) return -7;
-      hn[a] = b;
-    }
+  {
+    const int64_t ne = 3 * static_cast<int64_t>(T);
+    const int nc = rmp::chunks(ne);
+    std::vector<std::vector<int32_t>> open_e(static_cast<size_t>(nc));
+    rmp::for_chunks(ne, nc, [&](int k, int64_t b0, int64_t e0) {
+      for (int64_t e = b0; e < e0; ++e)
+        if (half[e] < 0) open_e[k].push_back(static_cast<int32_t>(e));
+    });
+    for (const auto& v : open_e)
+      for (const int32_t e : v) {
+        const int32_t a = tri[e], b = tri[e - e % 3 + (e % 3 + 1) % 3];
+        if (hn[a] >= 0) return -7;
+        hn[a] = b;
+      }
+  }
   for (int32_t a = 0; a < n; ++a) {
     if (hn[a] < 0) continue;
     const int32_t b = hn[a], c = hn[b];

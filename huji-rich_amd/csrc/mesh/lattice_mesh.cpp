@@ -849,17 +849,23 @@ extern "C" rm_mover* rm_mover_create(const rm_box_points_params* prm) {
 extern "C" rm_mesh* rm_mover_mesh(rm_mover* mv, const double* x, const double* y) {
   if (!mv || !x || !y) return nullptr;
   const int32_t N = mv->N;
-  for (int32_t i = 0; i < N; ++i)
-    if (!std::isfinite(x[i]) || !std::isfinite(y[i])) return nullptr;
-  bool ok = !mv->tri.empty();
-  // every point near a wall still has its reflection
-  for (int w = 0; ok && w < 4; ++w)
-    for (int32_t i = 0; i < N; ++i)
+  // finite points; with a triangulation kept, every point near a wall
+  // still has its reflection
+  const bool kept = !mv->tri.empty();
+  int nonfinite = 0, unreflected = 0;
+  rmp::parallel_for(N, [&](int64_t i) {
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i])) {
+      __atomic_store_n(&nonfinite, 1, __ATOMIC_RELAXED);
+      return;
+    }
+    if (!kept) return;
+    for (int w = 0; w < 4; ++w)
       if (!mv->has[static_cast<size_t>(w) * N + i] &&
-          near_wall(w, x[i], y[i], mv->x0, mv->x1, mv->y0, mv->y1, mv->margin)) {
-        ok = false;
-        break;
-      }
+          near_wall(w, x[i], y[i], mv->x0, mv->x1, mv->y0, mv->y1, mv->margin))
+        __atomic_store_n(&unreflected, 1, __ATOMIC_RELAXED);
+  });
+  if (nonfinite) return nullptr;
+  bool ok = kept && !unreflected;
   if (ok) {  // the frame stays where it is
     std::copy(x, x + N, mv->X.begin());
     std::copy(y, y + N, mv->Y.begin());
