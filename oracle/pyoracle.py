@@ -7,7 +7,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# RICH_ORACLE_LIB names another build of rich_oracle.c (the mutation test
+# loads copies with planted mistakes)
+LIB_PATH = os.environ.get("RICH_ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 TANGENT_PATH = os.path.join(_HERE, "build", "liboracle_tangent.so")
 
 CODES = {

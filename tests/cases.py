@@ -1,5 +1,7 @@
 """Initial conditions of the workloads (Sod, Sedov-Taylor, Kelvin-Helmholtz)
 and randomised states used by the tests, on flat lattice meshes."""
+from dataclasses import replace
+
 import numpy as np
 
 from rich_amd import mesh as rmesh
@@ -91,3 +93,69 @@ def small_meshes():
         ("periodic_flip", rmesh.lattice(16, 24, (0.0, 1.0, 0.0, 1.5), jitter=2e-2, seed=6,
                                         bc=rmesh.PERIODIC, flip_seed=7)),
     ]
+
+
+def merged_ghost_mesh(n=14, seed=21, pairs=6):
+    """A rigid-box mesh in which some ghosts border two real cells.
+
+    In RICH's rigid boxes each ghost mirrors one cell and borders only it,
+    so RigidWallGenerator's 'real side of the ghost's first boundary face'
+    and GetOriginalIndex name the same cell.  To tell the two readings
+    apart, the ghost side of a lower-numbered wall face is renamed to the
+    ghost of a neighbouring wall cell: that ghost then borders two real
+    cells, and its first face's real side is not its original."""
+    m = rmesh.lattice(n, n, (0.0, 1.0, 0.0, 1.0), jitter=3e-2, seed=seed, bc=rmesh.RIGID)
+    N = m.n_cells
+    ff, fs = m.face_first.copy(), m.face_second.copy()
+    ghost_face = {}  # ghost -> its (only) face
+    for f in range(m.n_faces):
+        for g in (int(ff[f]), int(fs[f])):
+            if g >= N:
+                ghost_face.setdefault(g, []).append(f)
+    wall = {}  # real cell -> [(face, ghost)]
+    for g, fl in ghost_face.items():
+        for f in fl:
+            r = int(fs[f]) if int(ff[f]) == g else int(ff[f])
+            wall.setdefault(r, []).append((f, g))
+    used, merged = set(), []
+
+    def merge(a, b):
+        for (fa, ga) in wall[a]:
+            for (fb, gb) in wall[b]:
+                if {ga, gb} & used or len(ghost_face[ga]) != 1 or len(ghost_face[gb]) != 1:
+                    continue
+                lo, hi = (fa, fb) if fa < fb else (fb, fa)
+                g_lo, g_hi = (ga, gb) if lo == fa else (gb, ga)
+                # the lower face now names the other cell's ghost
+                if int(ff[lo]) == g_lo:
+                    ff[lo] = g_hi
+                else:
+                    fs[lo] = g_hi
+                used.update((ga, gb))
+                merged.append((g_hi, lo, hi))
+                return
+
+    for f in range(m.n_faces):
+        a, b = int(ff[f]), int(fs[f])
+        if a < N and b < N and a in wall and b in wall:
+            merge(a, b)
+        if len(merged) >= pairs:
+            break
+    assert len(merged) >= 2
+    m2 = replace(m, face_first=ff, face_second=fs, _handle=None)
+    return m2, merged
+
+
+def tiny_velocity_cells(m, shocked, seed=0):
+    """Velocities -1 (x) and +1 (y) with variations of 1e-11 between cells:
+    every face change of a velocity lies inside the 1e-9 guards.  Pressure
+    uniform (no shock anywhere) or alternating 1 / 2 on the lattice's
+    checkerboard (pressure ratio 0.5 < 0.7 at every cell: all shocked)."""
+    rng = np.random.default_rng(seed)
+    N = m.n_cells
+    c = np.zeros((4, N))
+    c[0] = 1.0 + 0.1 * np.sin(5 * m.cm_x[:N]) * np.cos(3 * m.cm_y[:N])
+    c[1] = 1.0 + ((m.lat_i + m.lat_j) % 2 if shocked else 0.0)
+    c[2] = -1.0 + 1e-11 * rng.random(N)
+    c[3] = 1.0 + 1e-11 * rng.random(N)
+    return c

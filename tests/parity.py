@@ -7,17 +7,26 @@ def compare(gpu, ref, rtol=1e-12):
 
     Scaled error of an entry = |g - r| / max(|r|, 1e-3 * max|r_row|): plain
     relative error, except that entries which cancel to near zero are
-    measured against their field's magnitude."""
+    measured against their field's magnitude.  Non-finite entries must match
+    exactly: NaN against NaN and an infinity against the same infinity count
+    as equal; NaN or an infinity against anything else is an infinite
+    error."""
     g = np.atleast_2d(np.asarray(gpu, dtype=np.float64))
     r = np.atleast_2d(np.asarray(ref, dtype=np.float64))
     assert g.shape == r.shape, (g.shape, r.shape)
-    rowmax = np.abs(r).max(axis=-1, keepdims=True)
-    scale = np.maximum(np.abs(r), 1e-3 * rowmax)
+    if g.size == 0:
+        return 0.0, 1.0
+    fin_r = np.where(np.isfinite(r), r, 0.0)
+    rowmax = np.abs(fin_r).max(axis=-1, keepdims=True)
+    scale = np.maximum(np.abs(fin_r), 1e-3 * rowmax)
     scale = np.where(scale > 0, scale, 1.0)
-    err = np.abs(g - r) / scale
-    err = np.where(np.isnan(g) & np.isnan(r), 0.0, err)
+    finite = np.isfinite(g) & np.isfinite(r)
+    with np.errstate(invalid="ignore", over="ignore"):
+        err = np.where(finite, np.abs(g - r) / scale, np.inf)
+    same_nonfinite = (np.isnan(g) & np.isnan(r)) | (np.isinf(g) & (g == r))
+    err = np.where(same_nonfinite, 0.0, err)
     same = (g.view(np.int64) == r.view(np.int64)).mean()
-    return float(np.nanmax(err)) if err.size else 0.0, float(same)
+    return float(np.max(err)), float(same)
 
 
 def assert_parity(gpu, ref, rtol=1e-12, what=""):
