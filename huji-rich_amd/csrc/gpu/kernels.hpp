@@ -16,6 +16,7 @@
 //             shock detector, both limiters); writes a 64-byte record of
 //             the limited x- and y-slopes read by the face kernel
 //   k_flux2   one thread per face: face states, rotated HLLC, rotate back
+//   k_flux_pcm one thread per face: ModularFluxCalculator over PCM
 //   k_flux1   one thread per face: first-order SimpleFluxCalculator
 //   k_update  one thread per cell: ascending-face segmented sum of
 //             -/+ dt*A*F, ZeroForce, Heun average, primitive recovery and
@@ -621,9 +622,13 @@ __device__ __forceinline__ void face_states(const DMesh& m, const D4* q, const R
   rt = interp_off(v[1], sxb, syb, fr.oxb, fr.oyb);
 }
 
-// With dt > 0 the face writes its extensive update dt*area*F (the products
-// SimpleExtensiveUpdater forms, simple_extensive_updater.cpp:23-34) instead
-// of F.
+// ModularFluxCalculator (modular_flux_calculator.cpp:39-71) on one face,
+// over LinearGaussImproved's face states or, with PCM, over PCM's
+// (interpolations/PCM.cpp:5-37: the cell on each side, a ghost side as its
+// ghost generator fills it).  With dt > 0 the face writes its extensive
+// update dt*area*F (the products SimpleExtensiveUpdater forms,
+// simple_extensive_updater.cpp:23-34) instead of F.
+template <bool PCM = false>
 __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict__ q,
                                            const Rec* __restrict__ rec, D4* __restrict__ flux,
                                            const DParams& prm, const DErr& err, int f,
@@ -631,7 +636,15 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
   Cell lf, rt;
-  face_states(m, q, rec, m.ff[f], m.fs[f], fr, lf, rt);
+  if (PCM) {
+    const int id[2] = {m.ff[f], m.fs[f]};
+    Cell v[2];
+    fetch_k(m, q, id, v);
+    lf = v[0];
+    rt = v[1];
+  } else {
+    face_states(m, q, rec, m.ff[f], m.fs[f], fr, lf, rt);
+  }
   const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
   // modular_flux_calculator.cpp:56-67
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
@@ -659,6 +672,14 @@ __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q
       dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*dt_slot)) : 0.0;
   flux2_face(m, q, rec, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x, dt);
 }
+// ModularFluxCalculator over PCM; dt_slot as for k_flux2
+__global__ void __launch_bounds__(256) k_flux_pcm(DMesh m, const D4* __restrict__ q,
+                                                  D4* __restrict__ flux, DParams prm, DErr err,
+                                                  const unsigned long long* __restrict__ dt_slot) {
+  const double dt =
+      dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*dt_slot)) : 0.0;
+  flux2_face<true>(m, q, nullptr, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x, dt);
+}
 // Persistent form: a fixed grid walks the faces in tiles of blockDim.x.
 __global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__ q,
                                                  const Rec* __restrict__ rec,
@@ -676,6 +697,18 @@ __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restri
   const FaceFrame fr = m.fframe[f];
   const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
   const Cell rt = side_state(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
+  left[f] = D4{lf.d, lf.p, lf.vx, lf.vy};
+  right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
+}
+
+// PCM::operator() (interpolations/PCM.cpp:5-37): res[f].first / .second are
+// the cells on the two sides, ghost sides as the ghost generator fills them
+__global__ void __launch_bounds__(256) k_face_states_pcm(DMesh m, const D4* __restrict__ q,
+                                                         D4* left, D4* right) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  const Cell lf = fetch(m, q, m.ff[f]);
+  const Cell rt = fetch(m, q, m.fs[f]);
   left[f] = D4{lf.d, lf.p, lf.vx, lf.vy};
   right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
 }

@@ -43,6 +43,7 @@ enum KernelId : int {
   KID_ADJ_CELL,
   KID_ENS,
   KID_MOVING,
+  KID_FLUX_PCM,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -51,7 +52,7 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
                                              "k_adj_slopes",  "k_adj_cell",     "k_ens",
-                                             "k_moving"};
+                                             "k_moving",      "k_flux_pcm"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -363,7 +364,7 @@ hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out 
 }
 
 // FluxCalculator on `cells` into c->flux (slopes into `rec`, default c->rec).
-// With a dt slot, second order writes dt*area*F (for UPD_DELTA).
+// With a dt slot, the modular calculators write dt*area*F (for UPD_DELTA).
 hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* rec = nullptr,
                     const unsigned long long* dt_slot = nullptr) {
   if (c->prm.order == 2) {
@@ -375,6 +376,11 @@ hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* r
                                                            derr(c, step, phase_base + 1), dt_slot);
     });
   }
+  if (c->prm.order == 3)
+    return launch(c, KID_FLUX_PCM, [&] {
+      k_flux_pcm<<<blocks_for(c->F), kBlock, 0, c->stream>>>(
+          c->dm, cells, c->flux, dparams(c), derr(c, step, phase_base + 1), dt_slot);
+    });
   return launch(c, KID_FLUX1, [&] {
     k_flux1<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, c->flux, dparams(c),
                                                          derr(c, step, phase_base + 1));
@@ -446,8 +452,8 @@ HeunBufs ctx_bufs(rg_ctx* c) {
 // CFL terms into slot (s+1)&1.  dt_hist[hist] receives the step's dt.
 hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs& h) {
   const int cur = s & 1, nxt = (s + 1) & 1;
-  // second order writes dt*area*F straight from the flux kernel
-  const int delta = c->prm.order == 2 ? UPD_DELTA : 0;
+  // the modular calculators write dt*area*F straight from the flux kernel
+  const int delta = c->prm.order >= 2 ? UPD_DELTA : 0;
   const unsigned long long* slot = delta ? c->words + cur : nullptr;
   hipError_t e = run_flux(c, h.P, s, 1, h.r1, slot);
   if (e != hipSuccess) return e;
@@ -690,7 +696,7 @@ extern "C" {
 
 int rg_create(const rg_params* params, rg_ctx** out) {
   if (!params || !out) return RG_E_ARG;
-  if (!(params->gamma > 1.0) || !(params->cfl > 0.0) || (params->order != 1 && params->order != 2) ||
+  if (!(params->gamma > 1.0) || !(params->cfl > 0.0) || (params->order < 1 || params->order > 3) ||
       (params->bc != RG_BC_RIGID && params->bc != RG_BC_PERIODIC) ||
       (params->precision != RG_FP64 && params->precision != RG_MIXED))
     return RG_E_ARG;
@@ -1158,21 +1164,28 @@ int rg_time_step(rg_ctx* c, double* dt) {
 }
 
 int rg_reconstruct(rg_ctx* c, double* slopes, double* left, double* right) {
-  if (!ready(c, true, false) || c->prm.order != 2) return RG_E_ARG;
-  HIPCHK(c, run_slopes(c, c->prim, 0, 1));
+  if (!ready(c, true, false)) return RG_E_ARG;
+  const bool pcm = c->prm.order != 2;
+  if (!pcm) HIPCHK(c, run_slopes(c, c->prim, 0, 1));
   if (left || right) {
     if (!c->left) {
       HIPCHK(c, dalloc(c, &c->left, c->F));
       HIPCHK(c, dalloc(c, &c->right, c->F));
     }
     HIPCHK(c, launch(c, KID_FACE_STATES, [&] {
-      k_face_states<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->prim, c->rec, c->left,
-                                                               c->right);
+      if (pcm)
+        k_face_states_pcm<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->prim, c->left,
+                                                                      c->right);
+      else
+        k_face_states<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, c->prim, c->rec,
+                                                                 c->left, c->right);
     }));
   }
   const int rc = collect_error(c);
   if (rc != RG_OK) return rc;
-  if (slopes) {
+  if (slopes && pcm) {
+    std::memset(slopes, 0, sizeof(double) * 8 * static_cast<size_t>(c->N));
+  } else if (slopes) {
     std::vector<Rec> h(static_cast<size_t>(c->N));
     HIPCHK(c, hipMemcpyAsync(h.data(), c->rec, sizeof(Rec) * c->N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -68,7 +68,8 @@ typedef struct rg_params {
   double gamma;    /* IdealGas(gamma)                      ideal_gas.cpp:5-6 */
   double cfl;      /* SimpleCFL(cfl)                       simple_cfl.cpp:6   */
   int32_t order;   /* 1: SimpleFluxCalculator (piecewise constant)
-                      2: ModularFluxCalculator + LinearGaussImproved          */
+                      2: ModularFluxCalculator + LinearGaussImproved
+                      3: ModularFluxCalculator + PCM (PCM.cpp:5-37)           */
   int32_t bc;      /* RG_BC_RIGID / RG_BC_PERIODIC                            */
   int32_t slf;     /* LinearGaussImproved slf        LinearGaussImproved.hpp:31-40 */
   double delta_v;  /* shock detector |div v|*L/c       (default 0.2)         */
@@ -158,7 +159,9 @@ int rg_init_extensives(rg_ctx* ctx);
 int rg_time_step(rg_ctx* ctx, double* dt);
 /* SpatialReconstruction: slopes [8][N] (d/dx of density, pressure, vx, vy,
  * then d/dy) and face states res[f].first/.second as left/right [4][F]
- * (density, pressure, vx, vy).  Any output may be NULL. */
+ * (density, pressure, vx, vy).  Any output may be NULL.  Order 2 is
+ * LinearGaussImproved; orders 1 and 3 give PCM's states (PCM.cpp:5-37)
+ * and zero slopes. */
 int rg_reconstruct(rg_ctx* ctx, double* slopes, double* left, double* right);
 /* FluxCalculator: fluxes per unit face length, kept on the device for
  * rg_update_extensives; copied out when the pointers are non-NULL [F]. */

@@ -74,6 +74,7 @@ def lib():
         L.ro_lgi.restype = Error_
         L.ro_lgi.argtypes = [P(Mesh_), P(Params_), Quad_, ctypes.c_void_p, ctypes.c_void_p,
                              Quad_, Quad_]
+        L.ro_pcm.argtypes = [P(Mesh_), P(Params_), Quad_, Quad_, Quad_]
         L.ro_flux.restype = Error_
         L.ro_flux.argtypes = [P(Mesh_), P(Params_), Quad_, ctypes.c_void_p, ctypes.c_void_p,
                               Quad_]
@@ -170,6 +171,14 @@ class Oracle:
                          _quad(L), _quad(R))
         self._check(e)
         return s, nv, L, R
+
+    def pcm(self, cells):
+        """PCM: (left (4,F), right (4,F)) face states."""
+        c = self._cells_ng(cells)
+        L = np.zeros((4, self.F))
+        R = np.zeros((4, self.F))
+        lib().ro_pcm(ctypes.byref(self.m), ctypes.byref(self.prm), _quad(c), _quad(L), _quad(R))
+        return L, R
 
     def flux(self, cells, w=None):
         c = self._cells_ng(cells)

@@ -679,6 +679,16 @@ ro_error ro_lgi(const ro_mesh* m, const ro_params* prm, ro_cells cng, real* cons
   return err;
 }
 
+/* PCM::operator() (interpolations/PCM.cpp:5-37): each side's state is the
+ * cell itself, or for a ghost side the ghost generator's value. */
+void ro_pcm(const ro_mesh* m, const ro_params* prm, ro_cells cng, ro_cells left, ro_cells right) {
+  ro_ghost_cells(m, prm, cng);
+  for (int32_t f = 0; f < m->n_faces; ++f) {
+    put_cell(left, f, get_cell(cng, m->face_first[f]));
+    put_cell(right, f, get_cell(cng, m->face_second[f]));
+  }
+}
+
 /* ------------------------------------------------------ flux calculators */
 static int modular_face_flux(const ro_mesh* m, const ro_params* prm, int32_t f, const cc_t* lf,
                              const cc_t* rt, double wx, double wy, cons_t* out) {
@@ -759,11 +769,8 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
   real* lbuf = NULL;
   (void)N;
   (void)NG;
-  if (prm->order == 2) {
-    sbuf = (real*)malloc(sizeof(real) * 8 * (size_t)N);
+  if (prm->order == 2 || prm->order == 3) {
     lbuf = (real*)malloc(sizeof(real) * 8 * (size_t)F);
-    real* slopes[8];
-    for (int k = 0; k < 8; ++k) slopes[k] = sbuf + (size_t)k * N;
     left.d = lbuf;
     left.p = lbuf + F;
     left.vx = lbuf + 2 * (size_t)F;
@@ -772,11 +779,22 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
     right.p = lbuf + 5 * (size_t)F;
     right.vx = lbuf + 6 * (size_t)F;
     right.vy = lbuf + 7 * (size_t)F;
+  }
+  if (prm->order == 2) {
+    sbuf = (real*)malloc(sizeof(real) * 8 * (size_t)N);
+    real* slopes[8];
+    for (int k = 0; k < 8; ++k) slopes[k] = sbuf + (size_t)k * N;
     const ro_error e = ro_lgi(m, prm, cng, slopes, NULL, left, right);
     if (e.code != RO_OK) {
       free(sbuf);
       free(lbuf);
       return e;
+    }
+  } else if (prm->order == 3) {
+    /* ModularFluxCalculator over PCM (modular_flux_calculator.cpp:39-71) */
+    for (int32_t f = 0; f < F; ++f) {
+      put_cell(left, f, get_cell(cng, m->face_first[f]));
+      put_cell(right, f, get_cell(cng, m->face_second[f]));
     }
   }
   int32_t first_bad = F;
@@ -787,7 +805,7 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
     const double w_x = wx ? wx[f] : 0.0, w_y = wy ? wy[f] : 0.0;
     cons_t c;
     int rc;
-    if (prm->order == 2) {
+    if (prm->order == 2 || prm->order == 3) {
       const cc_t lf = get_cell(left, f), rt = get_cell(right, f);
       rc = modular_face_flux(m, prm, f, &lf, &rt, w_x, w_y, &c);
     } else {

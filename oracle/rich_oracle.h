@@ -47,7 +47,8 @@ typedef struct ro_mesh {
 typedef struct ro_params {
   double gamma;   /* IdealGas adiabatic index                         */
   double cfl;     /* SimpleCFL                                        */
-  int32_t order;  /* 1: SimpleFluxCalculator, 2: Modular + LGI        */
+  int32_t order;  /* 1: SimpleFluxCalculator, 2: Modular + LGI,
+                     3: Modular + PCM                                  */
   int32_t bc;     /* 0 RigidWallGenerator, 1 PeriodicGhostGenerator   */
   int32_t slf;    /* LinearGaussImproved slope limiter flag            */
   double delta_v, theta, delta_p;
@@ -87,9 +88,14 @@ void ro_ghost_cells(const ro_mesh* m, const ro_params* prm, ro_cells cells_ng);
 ro_error ro_lgi(const ro_mesh* m, const ro_params* prm, ro_cells cells_ng,
                 double* const slopes[8], double* const naive[8],
                 ro_cells left, ro_cells right);
-/* FluxCalculator: ModularFluxCalculator (order 2) or SimpleFluxCalculator
- * (order 1).  cells_ng has N+G entries (ghost slots are filled here).
- * w: face velocities [2][F] (NULL = zero).  flux [4][F] per unit length. */
+/* PCM::operator(): face states [4][F] are the cells on each side, ghost
+ * sides from the ghost generator (cng's ghost slots, filled here). */
+void ro_pcm(const ro_mesh* m, const ro_params* prm, ro_cells cells_ng, ro_cells left,
+            ro_cells right);
+/* FluxCalculator: ModularFluxCalculator over LinearGaussImproved (order 2)
+ * or over PCM (order 3), or SimpleFluxCalculator (order 1).  cells_ng has
+ * N+G entries (ghost slots are filled here).  w: face velocities [2][F]
+ * (NULL = zero).  flux [4][F] per unit length. */
 ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cells_ng,
                  const double* wx, const double* wy, ro_ext flux);
 /* SimpleExtensiveUpdater. */

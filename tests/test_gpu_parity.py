@@ -32,7 +32,24 @@ def test_reconstruct(name):
           assert_parity(R, Ro, what="right"))
 
 
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 3])
+@pytest.mark.parametrize("name", list(MESHES))
+def test_reconstruct_pcm(name, order):
+    """PCM (PCM.cpp:5-37): face states are the cells and ghost values."""
+    m = MESHES[name]
+    cells = cases.rough_cells(m, seed=1)
+    g = _gpu(m, order=order)
+    g.set_cells(cells)
+    s, L, R = g.reconstruct()
+    Lo, Ro = po.Oracle(m, order=3).pcm(cells)
+    assert not s.any()
+    assert_parity(L, Lo, what="left")
+    assert_parity(R, Ro, what="right")
+    np.testing.assert_array_equal(L, Lo)
+    np.testing.assert_array_equal(R, Ro)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("name", list(MESHES))
 def test_flux(name, order):
     m = MESHES[name]
@@ -82,7 +99,7 @@ def test_time_step(name):
     assert g.time_step() == o.cfl(cells, w)
 
 
-@pytest.mark.parametrize("integrator,order", [(1, 1), (2, 2), (1, 2), (2, 1)])
+@pytest.mark.parametrize("integrator,order", [(1, 1), (2, 2), (1, 2), (2, 1), (1, 3), (2, 3)])
 @pytest.mark.parametrize("name", list(MESHES))
 def test_advance(name, integrator, order):
     m = MESHES[name]

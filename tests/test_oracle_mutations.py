@@ -17,7 +17,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "rich_oracle.c")
 PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_scheme.py",
-             "tests/test_oracle_hllc.py"]
+             "tests/test_oracle_hllc.py", "tests/test_oracle_pcm.py"]
 
 # name -> [(text in rich_oracle.c, replacement)], each text found exactly once
 MUTATIONS = {
@@ -50,6 +50,16 @@ MUTATIONS = {
     "rigid ghost from GetOriginalIndex": [("cc_t t = get_cell(c, side == 2 ? a : b);",
                                            "cc_t t = get_cell(c, orig_index(m, g));")],
     "rigid ghost from its last face": [("    if (filled[g - N]) continue;\n", "")],
+    # ModularFluxCalculator over PCM (PCM.cpp:5-37)
+    "PCM sides swapped in the flux": [
+        ("/* ModularFluxCalculator over PCM (modular_flux_calculator.cpp:39-71) */\n"
+         "    for (int32_t f = 0; f < F; ++f) {\n"
+         "      put_cell(left, f, get_cell(cng, m->face_first[f]));\n"
+         "      put_cell(right, f, get_cell(cng, m->face_second[f]));",
+         "/* ModularFluxCalculator over PCM (modular_flux_calculator.cpp:39-71) */\n"
+         "    for (int32_t f = 0; f < F; ++f) {\n"
+         "      put_cell(left, f, get_cell(cng, m->face_second[f]));\n"
+         "      put_cell(right, f, get_cell(cng, m->face_first[f]));")],
     # PeriodicGhostGenerator::GetGhostGradient (PeriodicGhostGenerator.cpp:19-24)
     "zero periodic ghost gradient": [("  if (prm->bc == 0) return z;\n", "  return z;\n")],
 }
