@@ -245,6 +245,8 @@ class RichGPU:
         """rg_advance(nsteps, Heun) keeping device checkpoints every `every`
         steps for adjoint_backward; needs init_extensives() of the cells."""
         dts = np.empty(max(nsteps, 1)) if record_dt else None
+        # the device keeps one recorded run: a later forward replaces it
+        self.adjoint_generation = getattr(self, "adjoint_generation", 0) + 1
         self._check(self._lib.rg_adjoint_forward(self._h, nsteps, every, _p(dts)))
         return dts[:nsteps] if record_dt else None
 

@@ -114,14 +114,34 @@ def correct_overshoot(mesh, vx, vy, dt, box):
     return vx, vy
 
 
+class HostMotion:
+    """The point motion of the run on the host: Lagrangian + RoundCells
+    point velocities, the overshoot fix and StationaryBox face velocities."""
+
+    def __init__(self, gamma=5.0 / 3.0):
+        self.gamma = gamma
+
+    def point_velocity(self, mesh, cells):
+        dwx, dwy = round_cells_dw(mesh, cells, self.gamma)
+        return cells[2] + dwx, cells[3] + dwy
+
+    def point_fix(self, mesh, vx, vy, dt, box):
+        return correct_overshoot(mesh, vx, vy, dt, box)
+
+    def face_velocity(self, mesh, vx, vy):
+        return mesh.face_velocity(vx, vy)
+
+
 class MovingMesh:
     """A moving-mesh run: points, cells (4, N) and extensives on the host
-    between steps; `step()` advances one TimeAdvance2Heun step."""
+    between steps; `step()` advances one TimeAdvance2Heun step.  `motion`
+    supplies the point and face velocities (default HostMotion)."""
 
-    def __init__(self, x, y, cells, box, backend, gamma=5.0 / 3.0):
+    def __init__(self, x, y, cells, box, backend, gamma=5.0 / 3.0, motion=None):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.b, self.gamma = box, backend, gamma
+        self.motion = motion if motion is not None else HostMotion(gamma)
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
         self.b.set_mesh(self.mesh)
@@ -140,14 +160,14 @@ class MovingMesh:
         b, T = self.b, self.timing
         t = time.perf_counter()
         # Lagrangian + RoundCells point velocities, StationaryBox face velocities
-        dwx, dwy = round_cells_dw(self.mesh, self.cells, self.gamma)
-        vx, vy = self.cells[2] + dwx, self.cells[3] + dwy
-        fw = self.mesh.face_velocity(vx, vy)
+        mo = self.motion
+        vx, vy = mo.point_velocity(self.mesh, self.cells)
+        fw = mo.face_velocity(self.mesh, vx, vy)
         T["motion_s"] += time.perf_counter() - t
         dt = self._plug(b.cfl, self.cells, fw)
         t = time.perf_counter()
-        vx, vy = correct_overshoot(self.mesh, vx, vy, dt, self.box)  # ApplyFix
-        fw = self.mesh.face_velocity(vx, vy)
+        vx, vy = mo.point_fix(self.mesh, vx, vy, dt, self.box)  # ApplyFix
+        fw = mo.face_velocity(self.mesh, vx, vy)
         T["motion_s"] += time.perf_counter() - t
         dt = self._plug(b.cfl, self.cells, fw)
         # stage 1 on the old mesh; ZeroForce adds dt*0 (hydrodynamics_2d.cpp:348-372)
@@ -167,7 +187,7 @@ class MovingMesh:
         self._plug(b.set_mesh, self.mesh)
         mid_cells = self._plug(b.update_cells, mid)
         t = time.perf_counter()
-        fw = self.mesh.face_velocity(vx, vy)
+        fw = mo.face_velocity(self.mesh, vx, vy)
         T["motion_s"] += time.perf_counter() - t
         # stage 2 on the new mesh
         flux = self._plug(b.flux, mid_cells, fw)

@@ -75,6 +75,14 @@ def lib():
         L.ro_lgi.argtypes = [P(Mesh_), P(Params_), Quad_, ctypes.c_void_p, ctypes.c_void_p,
                              Quad_, Quad_]
         L.ro_pcm.argtypes = [P(Mesh_), P(Params_), Quad_, Quad_, Quad_]
+        for n in ("ro_point_velocity", "ro_point_fix", "ro_face_velocity"):
+            getattr(L, n).restype = None
+        L.ro_point_velocity.argtypes = [P(Mesh_), ctypes.c_double] + [ctypes.c_void_p] * 4 + [
+            ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.ro_point_fix.argtypes = [P(Mesh_), ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p]
+        L.ro_face_velocity.argtypes = [P(Mesh_), ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.ro_flux.restype = Error_
         L.ro_flux.argtypes = [P(Mesh_), P(Params_), Quad_, ctypes.c_void_p, ctypes.c_void_p,
                               Quad_]
@@ -179,6 +187,29 @@ class Oracle:
         R = np.zeros((4, self.F))
         lib().ro_pcm(ctypes.byref(self.m), ctypes.byref(self.prm), _quad(c), _quad(L), _quad(R))
         return L, R
+
+    def point_velocity(self, cells, chi=0.15, eta=0.02):
+        """Lagrangian + RoundCells point velocities (2, N)."""
+        c = np.ascontiguousarray(cells[:, :self.N], dtype=np.float64)
+        w = np.zeros((2, self.N))
+        lib().ro_point_velocity(ctypes.byref(self.m), self.prm.gamma, *[_p(c[k]) for k in range(4)],
+                                chi, eta, _p(w[0]), _p(w[1]))
+        return w
+
+    def point_fix(self, w, dt, box):
+        """CorrectPointsOverShoot of the box (x0, x1, y0, y1): new (2, N)."""
+        w = np.array(w, dtype=np.float64, order="C")
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        lib().ro_point_fix(ctypes.byref(self.m), dt, _p(b), _p(w[0]), _p(w[1]))
+        return w
+
+    def face_velocity(self, w):
+        """StationaryBox / CalcFaceVelocity face velocities (2, F)."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        fw = np.zeros((2, self.F))
+        lib().ro_face_velocity(ctypes.byref(self.m), self.mesh.bc, _p(w[0]), _p(w[1]),
+                               _p(fw[0]), _p(fw[1]))
+        return fw
 
     def flux(self, cells, w=None):
         c = self._cells_ng(cells)

@@ -1056,3 +1056,86 @@ ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int3
   free(cb);
   return err;
 }
+
+#ifndef RO_TANGENT
+/* ------------------------------------------------ moving-mesh inputs -- */
+/* Lagrangian::operator() (point_motions/lagrangian.cpp:3-13) with
+ * RoundCells::operator() adding calc_dw (round_cells.cpp:63-76, 107-120;
+ * cold = false): r the mesh point, s the CM, d = abs(s - r),
+ * R = GetWidth; zero if d < 0.9 eta R, else
+ * chi * max(dp2c, abs(v)) * (s - r) / R. */
+void ro_point_velocity(const ro_mesh* m, double gamma, const double* d, const double* p,
+                       const double* vx, const double* vy, double chi, double eta,
+                       double* wx, double* wy) {
+  for (int32_t i = 0; i < m->n_cells; ++i) {
+    const double sx = m->cm_x[i] - m->pt_x[i], sy = m->cm_y[i] - m->pt_y[i];
+    const double dist = vabs(sx, sy);
+    const double R = sqrt(m->tess_volume[i] / M_PI);
+    double dwx = 0, dwy = 0;
+    if (!(dist < 0.9 * eta * R)) {
+      const double c = smax(dp2c(gamma, d[i], p[i]), vabs(vx[i], vy[i]));
+      /* chi_*c*(s - r) / R: (chi*c) * vector, then each component / R */
+      const double k = chi * c;
+      dwx = k * sx / R;
+      dwy = k * sy / R;
+    }
+    /* res[i] += calc_dw(...) */
+    wx[i] = vx[i] + dwx;
+    wy[i] = vy[i] + dwy;
+  }
+}
+
+/* CorrectPointsOverShoot (round_cells.cpp:14-61) for a box that is not
+ * half-periodic, in the order right, left, up, down; each slowed
+ * velocity is the one the next test sees. */
+void ro_point_fix(const ro_mesh* m, double dt, const double box[4], double* wx, double* wy) {
+  const double x0 = box[0], x1 = box[1], y0 = box[2], y1 = box[3];
+  const double inv_dt = 1.0 / dt;
+  for (int32_t i = 0; i < m->n_cells; ++i) {
+    const double px = m->pt_x[i], py = m->pt_y[i];
+    const double R = sqrt(m->tess_volume[i] / M_PI);
+    double vx = wx[i], vy = wy[i];
+    const double dists[4] = {x1 - px, px - x0, y1 - py, py - y0};
+    for (int k = 0; k < 4; ++k) {
+      const int hit = k == 0 ? (vx * dt * 2 + px > x1)
+                    : k == 1 ? (vx * dt * 2 + px < x0)
+                    : k == 2 ? (vy * dt * 2 + py > y1)
+                             : (vy * dt * 2 + py < y0);
+      if (!hit) continue;
+      double factor = 0.25 * dists[k] * inv_dt / vabs(vx, vy);
+      if (R * 0.1 > dists[k]) factor *= -0.1;
+      vx = vx * factor;
+      vy = vy * factor;
+    }
+    wx[i] = vx;
+    wy[i] = vy;
+  }
+}
+
+/* StationaryBox::operator() (stationary_box.cpp:20-40): zero on a face
+ * whose sides have the same GetOriginalIndex, else
+ * VoronoiMesh::CalcFaceVelocity (VoronoiMesh.cpp:146-151):
+ * w' = ScalarProd(wl - wr, f - (rR + rL)/2) * (rR - rL) / pow(abs(rR - rL), 2),
+ * w = 0.5*(wl + wr) + w', f the face centroid. */
+void ro_face_velocity(const ro_mesh* m, int32_t bc, const double* wx, const double* wy,
+                      double* fwx, double* fwy) {
+  for (int32_t f = 0; f < m->n_faces; ++f) {
+    const int32_t a = m->face_first[f], b = m->face_second[f];
+    const int32_t oa = orig_index(m, a), ob = orig_index(m, b);
+    if (bc == 0 && oa == ob) {
+      fwx[f] = 0.0;
+      fwy[f] = 0.0;
+      continue;
+    }
+    const double wlx = wx[oa], wly = wy[oa], wrx = wx[ob], wry = wy[ob];
+    const double rLx = m->pt_x[a], rLy = m->pt_y[a], rRx = m->pt_x[b], rRy = m->pt_y[b];
+    double fx, fy;
+    face_centroid(m, f, &fx, &fy);
+    const double s = (wlx - wrx) * (fx - (rRx + rLx) / 2) + (wly - wry) * (fy - (rRy + rLy) / 2);
+    const double dx = rRx - rLx, dy = rRy - rLy;
+    const double a2 = pow(vabs(dx, dy), 2);
+    fwx[f] = 0.5 * (wlx + wrx) + s * dx / a2;
+    fwy[f] = 0.5 * (wly + wry) + s * dy / a2;
+  }
+}
+#endif

@@ -110,6 +110,21 @@ double ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells,
 /* hdsim init_extensives. */
 void ro_init_extensives(const ro_mesh* m, const ro_params* prm, ro_cells cells,
                         ro_ext ext);
+/* --- moving-mesh inputs (rigid box) --- */
+/* Lagrangian point velocities plus RoundCells::calc_dw without the cold
+ * switch: w_i = v_i + chi max(c_i, |v_i|) (CM_i - r_i) / R_i where
+ * |CM_i - r_i| >= 0.9 eta R_i, R = sqrt(GetVolume/pi).  [N] each. */
+void ro_point_velocity(const ro_mesh* m, double gamma, const double* d, const double* p,
+                       const double* vx, const double* vy, double chi, double eta,
+                       double* wx, double* wy);
+/* CorrectPointsOverShoot of the box [x0, x1, y0, y1] for step dt, in
+ * place on the point velocities [N]. */
+void ro_point_fix(const ro_mesh* m, double dt, const double box[4], double* wx, double* wy);
+/* Face velocities [F] from point velocities [N] (StationaryBox with
+ * CalcFaceVelocity; a ghost side moves with its original). */
+void ro_face_velocity(const ro_mesh* m, int32_t bc, const double* wx, const double* wy,
+                      double* fwx, double* fwy);
+
 /* Whole steps on a static mesh: hdsim::TimeAdvance (integrator 1) or
  * hdsim::TimeAdvance2Heun (integrator 2).  cells_ng [N+G], ext [N].
  * dt_out[step] receives each step's dt (may be NULL). */

@@ -130,3 +130,23 @@ def test_inverse_problem_gradient_against_cpu_reference():
         print("adjoint", an, "forward-mode CPU", ref)
         assert abs(an - ref) <= 1e-9 * abs(ref)
     P.close()
+
+
+def test_autograd_refuses_a_replaced_run():
+    """A context records one forward run: after a second forward on the same
+    context, the first graph's backward raises instead of returning the
+    second run's gradient."""
+    import torch
+    from rich_amd.autograd import advance
+    from rich_amd.gpu import RichGPU
+    m = MESHES["periodic"]
+    cells = smooth_cells(m)
+    g = RichGPU(m, order=2)
+    c0 = torch.from_numpy(cells).requires_grad_(True)
+    out1 = advance(c0, g, 4, 2)
+    out2 = advance(c0 * 1.0, g, 4, 2)
+    with pytest.raises(RuntimeError, match="another forward"):
+        out1.sum().backward()
+    out2.sum().backward()
+    assert torch.isfinite(c0.grad).all()
+    g.close()

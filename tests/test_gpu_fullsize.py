@@ -70,14 +70,14 @@ def test_kh_2048x1024_conservation_and_mixed_precision(kh):
 def test_moving_sedov_512x512_device_equals_oracle():
     """Two moving-mesh steps at full size: the device-resident step and the
     same driver over the oracle give the same points, dts and state."""
-    from rich_amd.moving import DeviceMovingMesh, MovingMesh
-    from test_gpu_moving import OracleBackend
+    from rich_amd.moving import DeviceMovingMesh
+    from oracle_drivers import oracle_moving_mesh
     m = cases.sedov_mesh(512)
     N = m.n_cells
     box = (-1.0, 1.0, -1.0, 1.0)
     c = cases.sedov_cells(m)
     dev = DeviceMovingMesh(m.pt_x[:N], m.pt_y[:N], c, box)
-    ref = MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, OracleBackend())
+    ref = oracle_moving_mesh(m.pt_x[:N], m.pt_y[:N], c, box)
     x0 = dev.extensives()
     for step in range(2):
         assert dev.step() == ref.step(), step

@@ -7,28 +7,9 @@ import pytest
 
 import cases
 import pyoracle as po
+from oracle_drivers import oracle_moving_mesh
 
 pytestmark = pytest.mark.gpu
-
-
-class OracleBackend:
-    def set_mesh(self, mesh):
-        self.o = po.Oracle(mesh, order=2)
-
-    def init_extensives(self, cells):
-        return self.o.init_extensives(cells)
-
-    def cfl(self, cells, fw):
-        return self.o.cfl(cells, np.array(fw))
-
-    def flux(self, cells, fw):
-        return self.o.flux(cells, np.array(fw))
-
-    def update_extensives(self, flux, dt, ext):
-        return self.o.update_extensives(flux, dt, ext)
-
-    def update_cells(self, ext):
-        return self.o.update_cells(ext)
 
 
 @pytest.mark.parametrize("n", [20, 32])
@@ -39,7 +20,7 @@ def test_moving_sedov_gpu_equals_oracle(n):
     N = m.n_cells
     box = (-1.0, 1.0, -1.0, 1.0)
     runs = [MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, GpuBackend()),
-            MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, OracleBackend())]
+            oracle_moving_mesh(m.pt_x[:N], m.pt_y[:N], c, box)]
     moved = 0.0
     for step in range(8):
         dts = [r.step() for r in runs]
@@ -59,14 +40,15 @@ def test_moving_sedov_gpu_equals_oracle(n):
 def test_moving_device_resident_equals_oracle(n):
     """rg_moving_begin / rg_replace_mesh / rg_moving_end (point and face
     velocities, the overshoot fix and both stages on the device) against the
-    host-driven run over the oracle: same points, dt and state every step."""
-    from rich_amd.moving import DeviceMovingMesh, MovingMesh
+    host-driven run over the oracle (plugins and point motion): same points,
+    dt and state every step."""
+    from rich_amd.moving import DeviceMovingMesh
     m = cases.sedov_mesh(n)
     c = cases.sedov_cells(m)
     N = m.n_cells
     box = (-1.0, 1.0, -1.0, 1.0)
     dev = DeviceMovingMesh(m.pt_x[:N], m.pt_y[:N], c, box)
-    ref = MovingMesh(m.pt_x[:N], m.pt_y[:N], c, box, OracleBackend())
+    ref = oracle_moving_mesh(m.pt_x[:N], m.pt_y[:N], c, box)
     for step in range(8):
         dt_dev, dt_ref = dev.step(), ref.step()
         print(f"n={n} step={step} dt={dt_dev:.6e}", flush=True)

@@ -43,8 +43,8 @@ def test_static_restart(tmp_path):
 
 
 def test_moving_restart(tmp_path):
-    from rich_amd.moving import DeviceMovingMesh, MovingMesh
-    from test_gpu_moving import OracleBackend
+    from rich_amd.moving import DeviceMovingMesh
+    from oracle_drivers import oracle_moving_mesh
     m = cases.sedov_mesh(24)
     N = m.n_cells
     box = (-1.0, 1.0, -1.0, 1.0)
@@ -56,7 +56,7 @@ def test_moving_restart(tmp_path):
     run.close()
     s = snapshot.read(p)
     a = DeviceMovingMesh(s["x"], s["y"], s["cells"], box)
-    b = MovingMesh(s["x"], s["y"], s["cells"], box, OracleBackend())
+    b = oracle_moving_mesh(s["x"], s["y"], s["cells"], box)
     for step in range(4):
         assert a.step() == b.step(), step
     np.testing.assert_array_equal(a.x, b.x)

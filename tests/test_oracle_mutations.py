@@ -17,7 +17,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "rich_oracle.c")
 PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_scheme.py",
-             "tests/test_oracle_hllc.py", "tests/test_oracle_pcm.py"]
+             "tests/test_oracle_hllc.py", "tests/test_oracle_pcm.py",
+             "tests/test_oracle_moving.py"]
 
 # name -> [(text in rich_oracle.c, replacement)], each text found exactly once
 MUTATIONS = {
@@ -60,6 +61,16 @@ MUTATIONS = {
          "    for (int32_t f = 0; f < F; ++f) {\n"
          "      put_cell(left, f, get_cell(cng, m->face_second[f]));\n"
          "      put_cell(right, f, get_cell(cng, m->face_first[f]));")],
+    # RoundCells (round_cells.cpp:14-76) and CalcFaceVelocity (VoronoiMesh.cpp:146-151)
+    "calc_dw threshold eta R": [("if (!(dist < 0.9 * eta * R))", "if (!(dist < eta * R))")],
+    "calc_dw without max(c, |v|)": [("smax(dp2c(gamma, d[i], p[i]), vabs(vx[i], vy[i]))",
+                                     "dp2c(gamma, d[i], p[i])")],
+    "overshoot never turns back": [("      if (R * 0.1 > dists[k]) factor *= -0.1;\n", "")],
+    "face velocity without the bisector term": [
+        ("fwx[f] = 0.5 * (wlx + wrx) + s * dx / a2;", "fwx[f] = 0.5 * (wlx + wrx) + 0 * s * dx / a2;"),
+        ("fwy[f] = 0.5 * (wly + wry) + s * dy / a2;", "fwy[f] = 0.5 * (wly + wry) + 0 * s * dy / a2;")],
+    "mirror faces move": [("    if (bc == 0 && oa == ob) {\n      fwx[f] = 0.0;",
+                           "    if (bc == 0 && oa == ob && 0) {\n      fwx[f] = 0.0;")],
     # PeriodicGhostGenerator::GetGhostGradient (PeriodicGhostGenerator.cpp:19-24)
     "zero periodic ghost gradient": [("  if (prm->bc == 0) return z;\n", "  return z;\n")],
 }
