@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <unordered_map>
@@ -73,6 +74,40 @@ struct TimedLaunch {
 
 constexpr int kBlock = 256;
 inline unsigned blocks_for(int n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+// Scratch of rg_advance_noalloc: [0], [1] the dt slots, [2] the error word,
+// [3] unused, then the CFL block minima (one per block of kBlock cells),
+// then the dt history.  1 MiB; each GPU holds its own copy until the
+// process ends and every call on that GPU shares it.  Host code never
+// touches `scratch` by name: it uses the device address that
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr).
+constexpr int kScratchWords = 4;
+constexpr int kScratchBlocks = 65536;  // cells: up to kScratchBlocks * kBlock
+constexpr int kScratchSteps = 65536;   // dt history entries
+static __device__ __align__(256) double scratch[kScratchWords + kScratchBlocks + kScratchSteps];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+hipError_t scratch_on_current_device(int* dev_out, double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *dev_out = dev;
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 }  // namespace
 
@@ -157,6 +192,10 @@ struct rg_ctx {
   D4* keep = nullptr;
   size_t keep_cap = 0;
   bool pv_valid = false;
+
+  // device of the context's stream and its copy of the noalloc scratch
+  int device = 0;
+  double* scratch = nullptr;
 
   bool prof = false;
   std::vector<TimedLaunch> pending;
@@ -690,6 +729,69 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
   return collect_error(c);
 }
 
+// The FP64 step loop of rg_advance on a static mesh: the dt slots, block
+// minima and dt history are the context's words, partials and dt_hist
+// (record: write each step's dt to dt_hist).
+hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record) {
+  // first dt: SimpleCFL on the current cells into slot 0
+  hipError_t e = run_cfl(c, c->prim, 0);
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+    const int cur = s & 1, nxt = (s + 1) & 1;
+    if (integrator == 1) {
+      // hdsim::TimeAdvance: fluxes, extensive update, ZeroForce, cell update
+      e = run_flux(c, c->prim, s, 1);
+      if (e != hipSuccess) break;
+      UpdArgs a{};
+      a.in = c->ext;
+      a.out = c->ext;
+      a.flux = c->flux;
+      a.cells = c->prim;
+      a.dt_slot = c->words + cur;
+      a.partials = c->partials;
+      a.dt_hist = record ? c->dt_hist : nullptr;
+      a.step = s;
+      a.flags = UPD_FACES | UPD_ZF_AFTER | UPD_CELLS | UPD_CFL;
+      e = run_update(c, a, s, 3);
+      if (e == hipSuccess) e = run_reduce_min(c, nxt);
+    } else {
+      e = heun_step(c, s, record ? c->dt_hist : nullptr, s, ctx_bufs(c));
+    }
+  }
+  return e;
+}
+
+// For the duration of a rg_advance_noalloc call: the context's dt slots,
+// error word, block minima and dt history are the device's scratch, and no
+// per-launch timing events are recorded.
+struct ScratchScope {
+  rg_ctx* c;
+  unsigned long long* words;
+  double* partials;
+  double* dt_hist;
+  bool prof;
+  explicit ScratchScope(rg_ctx* ctx)
+      : c(ctx), words(ctx->words), partials(ctx->partials), dt_hist(ctx->dt_hist),
+        prof(ctx->prof) {
+    c->words = reinterpret_cast<unsigned long long*>(c->scratch);
+    c->partials = c->scratch + kScratchWords;
+    c->dt_hist = c->scratch + kScratchWords + kScratchBlocks;
+    c->prof = false;
+  }
+  ~ScratchScope() {
+    c->words = words;
+    c->partials = partials;
+    c->dt_hist = dt_hist;
+    c->prof = prof;
+  }
+};
+
+int arg_error(rg_ctx* c, const char* msg) {
+  c->last = rg_error{};
+  c->last.code = -1;
+  std::snprintf(c->last.message, sizeof(c->last.message), "%s", msg);
+  return RG_E_ARG;
+}
+
 }  // namespace
 
 extern "C" {
@@ -711,6 +813,10 @@ int rg_create(const rg_params* params, rg_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->words), 4 * sizeof(unsigned long long));
   if (e == hipSuccess) e = fill_words(c, 0, 4, ~0ull);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  // rg_advance_noalloc's scratch on this device (a failure here only
+  // disables that entry point)
+  if (e == hipSuccess && scratch_on_current_device(&c->device, &c->scratch) != hipSuccess)
+    c->scratch = nullptr;
   if (e != hipSuccess) {
     rg_destroy(c);
     return RG_E_HIP;
@@ -1289,29 +1395,36 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
   }
   if (c->prm.precision == RG_MIXED && integrator == 2 && c->prm.order == 2)
     return advance_mixed(c, nsteps, dt_out);
-  // first dt: SimpleCFL on the current cells into slot 0
-  HIPCHK(c, run_cfl(c, c->prim, 0));
-  for (int s = 0; s < nsteps; ++s) {
-    const int cur = s & 1, nxt = (s + 1) & 1;
-    if (integrator == 1) {
-      // hdsim::TimeAdvance: fluxes, extensive update, ZeroForce, cell update
-      HIPCHK(c, run_flux(c, c->prim, s, 1));
-      UpdArgs a{};
-      a.in = c->ext;
-      a.out = c->ext;
-      a.flux = c->flux;
-      a.cells = c->prim;
-      a.dt_slot = c->words + cur;
-      a.partials = c->partials;
-      a.dt_hist = dt_out ? c->dt_hist : nullptr;
-      a.step = s;
-      a.flags = UPD_FACES | UPD_ZF_AFTER | UPD_CELLS | UPD_CFL;
-      HIPCHK(c, run_update(c, a, s, 3));
-      HIPCHK(c, run_reduce_min(c, nxt));
-    } else {
-      HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s, ctx_bufs(c)));
-    }
+  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr));
+  if (dt_out)
+    HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
+                             c->stream));
+  return collect_error(c);
+}
+
+int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
+  if (!ready(c, true, true) || (integrator != 1 && integrator != 2) || nsteps < 0)
+    return RG_E_ARG;
+  if (c->ens_K > 0 || c->prm.precision != RG_FP64)
+    return arg_error(c, "rg_advance_noalloc: FP64 contexts without instances only");
+  if (nsteps > kScratchSteps)
+    return arg_error(c, "rg_advance_noalloc: more than 65536 steps in one call");
+  if (blocks_for(c->N) > static_cast<unsigned>(kScratchBlocks))
+    return arg_error(c, "rg_advance_noalloc: more than 16777216 cells");
+  if (!c->scratch) {
+    c->last = rg_error{};
+    c->last.code = -1;
+    std::snprintf(c->last.message, sizeof(c->last.message),
+                  "rg_advance_noalloc: no scratch on this device");
+    return RG_E_HIP;
   }
+  if (nsteps == 0) return RG_OK;
+  c->ext_init = false;
+  ScratchScope scope(c);
+  // the scratch keeps the last call's contents: start with clear dt slots
+  // and error word (a fill kernel, no memset)
+  HIPCHK(c, fill_words(c, 0, 3, ~0ull));
+  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr));
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));

@@ -74,6 +74,7 @@ def lib():
     L.rg_update_extensives.argtypes = [vp, ctypes.c_double]
     L.rg_update_cells.argtypes = [vp]
     L.rg_advance.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
+    L.rg_advance_noalloc.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
     L.rg_adjoint_forward.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, vp]
     L.rg_set_instances.argtypes = [vp, ctypes.c_int32, vp]
     L.rg_ensemble_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
@@ -239,6 +240,13 @@ class RichGPU:
     def advance(self, nsteps, integrator=2, record_dt=True):
         dts = np.empty(max(nsteps, 1)) if record_dt else None
         self._check(self._lib.rg_advance(self._h, integrator, nsteps, _p(dts)))
+        return dts[:nsteps] if record_dt else None
+
+    def advance_noalloc(self, nsteps, integrator=2, record_dt=True):
+        """rg_advance_noalloc: rg_advance with no device allocation in the
+        call (scratch in the library's fixed device array)."""
+        dts = np.empty(max(nsteps, 1)) if record_dt else None
+        self._check(self._lib.rg_advance_noalloc(self._h, integrator, nsteps, _p(dts)))
         return dts[:nsteps] if record_dt else None
 
     def adjoint_forward(self, nsteps, every=20, record_dt=True):

@@ -177,6 +177,19 @@ int rg_update_cells(rg_ctx* ctx);
  * [nsteps] receives the dts (may be NULL).  Synchronises once at the end. */
 int rg_advance(rg_ctx* ctx, int32_t integrator, int32_t nsteps, double* dt_out);
 
+/* No counterpart in the reference (hdsim allocates as it goes): rg_advance
+ * for callers that cannot allocate device memory while they step -- code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, results and errors as rg_advance for an RG_FP64 context
+ * without instances (else RG_E_ARG); it calls no hipMalloc or hipFree.  The
+ * dt slots, CFL block minima and dt history live in one fixed device array
+ * of the library (1 MiB on each GPU it runs on, held until the process
+ * ends), so nsteps <= 65536 and n_cells <= 16777216 per call (else
+ * RG_E_ARG with a message in rg_last_error).  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (other host threads,
+ * other streams) race. */
+int rg_advance_noalloc(rg_ctx* ctx, int32_t integrator, int32_t nsteps, double* dt_out);
+
 /* adjoint of the fixed-topology step ----------------------------------- */
 /* No counterpart in the reference (RICH has no derivatives): the reverse
  * mode of rg_advance(ctx, 2, nsteps) for order 2, RG_FP64, zero face
