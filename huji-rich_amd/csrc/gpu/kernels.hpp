@@ -320,6 +320,7 @@ __global__ void __launch_bounds__(256) k_geo_cells(DMesh m, CellGeo* cg, double*
     const double det = M0 * M3 - M1 * M2;
     if (fabs(det) < 1e-10 * vol * vol) {
       g.kind = KIND_SINGULAR;
+      g.c0 = det;  // the UniversalError entry "Det was"
     } else {
       const double det_inv = 1.0 / det;
       g.kind = KIND_NAIVE;
@@ -713,31 +714,32 @@ __global__ void __launch_bounds__(256) k_face_states_pcm(DMesh m, const D4* __re
   right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
 }
 
-// SimpleFluxCalculator body (simple_flux_calculator.cpp:80-126,167-189):
-// normal = point difference; a mirror face reflects the real state about the
-// face (Reflect, geometry.cpp:117-120).
-__global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q,
-                                               D4* __restrict__ flux, DParams prm, DErr err) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= m.F) return;
+// The two states SimpleFluxCalculator solves on face f, with its normal
+// (the point difference) and face direction (simple_flux_calculator.cpp:
+// 80-126): a mirror face reflects the real state about the face (Reflect,
+// geometry.cpp:117-120).
+__device__ __forceinline__ void simple_face_states(const DMesh& m, const D4* __restrict__ q, int f,
+                                                   Cell& l, Cell& r, double& nx, double& ny,
+                                                   double& px, double& py) {
   const int a = m.ff[f], b = m.fs[f];
-  const double px = m.v2x[f] - m.v1x[f], py = m.v2y[f] - m.v1y[f];
-  const double nx = m.px[b] - m.px[a], ny = m.py[b] - m.py[a];
+  px = m.v2x[f] - m.v1x[f];
+  py = m.v2y[f] - m.v1y[f];
+  nx = m.px[b] - m.px[a];
+  ny = m.py[b] - m.py[a];
   const bool fa = a >= 0 && a < m.N, fb = b >= 0 && b < m.N;
   const int oa = fa ? a : m.gorig[a - m.N];
   const int ob = fb ? b : m.gorig[b - m.N];
   const bool mirror = oa == ob;  // GetOriginalIndex equal on both sides
   // Reflect (geometry.cpp:117-120) of a side's velocity about the face
   auto reflect = [&](const Cell& s) {
-    Cell r = s;
+    Cell o = s;
     const double s2 = 2 * (s.vx * px + s.vy * py);
     const double ap = vabs(px, py);
     const double a2 = ap * ap;
-    r.vx = px * s2 / a2 - s.vx;
-    r.vy = py * s2 / a2 - s.vy;
-    return r;
+    o.vx = px * s2 / a2 - s.vx;
+    o.vy = py * s2 / a2 - s.vy;
+    return o;
   };
-  Cell l, r;
   if (!fa) {
     r = ld_cell(q, b);
     l = mirror ? reflect(r) : fetch(m, q, a);
@@ -748,6 +750,16 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q
     l = ld_cell(q, a);
     r = ld_cell(q, b);
   }
+}
+
+// SimpleFluxCalculator body (simple_flux_calculator.cpp:80-126,167-189).
+__global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q,
+                                               D4* __restrict__ flux, DParams prm, DErr err) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= m.F) return;
+  Cell l, r;
+  double nx, ny, px, py;
+  simple_face_states(m, q, f, l, r, nx, ny, px, py);
   const double an = vabs(nx, ny), ap = vabs(px, py);
   const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
   const double velocity = (nx * wx + ny * wy) / an;
@@ -759,6 +771,57 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q
   else
     o = D4{c.m, c.E, c.mx, c.my};
   flux[f] = o;
+}
+
+// The entries of the reference's "Invalid wave speeds in hllc solver"
+// error (hllc.cpp:46-71) for face f, recomputed from the inputs of the
+// flux call that failed (cells q and, at second order, their slopes rec):
+// the states Hllc received (rotated into the face frame, then shifted by
+// the face speed), the face speed and the three wave speeds.  One thread.
+__global__ void k_face_entries(DMesh m, const D4* __restrict__ q, const Rec* __restrict__ rec,
+                               int order, double gamma, int f, double* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0 || f < 0 || f >= m.F) return;
+  Cell L, R;
+  double nx, ny, px, py;
+  if (order == 1) {
+    simple_face_states(m, q, f, L, R, nx, ny, px, py);
+  } else {
+    const FaceFrame fr = m.fframe[f];
+    if (order == 3) {
+      L = fetch(m, q, m.ff[f]);
+      R = fetch(m, q, m.fs[f]);
+    } else {
+      face_states(m, q, rec, m.ff[f], m.fs[f], fr, L, R);
+    }
+    nx = fr.nx;
+    ny = fr.ny;
+    px = fr.px;
+    py = fr.py;
+  }
+  const double an = vabs(nx, ny), ap = vabs(px, py);
+  const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
+  const double velocity = (nx * wx + ny * wy) / an;
+  const Cell* side[2] = {&L, &R};
+  double u[2], c[2];
+  for (int k = 0; k < 2; ++k) {
+    const Cell& S = *side[k];
+    c[k] = dp2c(gamma, S.d, S.p);
+    u[k] = (S.vx * nx + S.vy * ny) / an - 1.0 * velocity;
+    out[6 * k + 0] = S.d;
+    out[6 * k + 1] = S.p;
+    out[6 * k + 2] = u[k];
+    out[6 * k + 3] = (S.vx * px + S.vy * py) / ap - 0.0 * velocity;
+    out[6 * k + 4] = c[k];
+    out[6 * k + 5] = dp2e(gamma, S.d, S.p);
+  }
+  const double sl = smin(u[0] - c[0], u[1] - c[1]);
+  const double sr = smax(u[0] + c[0], u[1] + c[1]);
+  const double ss = (R.p - L.p + L.d * u[0] * (sl - u[0]) - R.d * u[1] * (sr - u[1])) /
+                    (L.d * (sl - u[0]) - R.d * (sr - u[1]));
+  out[12] = velocity;
+  out[13] = sl;
+  out[14] = ss;
+  out[15] = sr;
 }
 
 // ------------------------------------------------------------------- CFL --
@@ -828,6 +891,10 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
                                               const DParams& prm, const DErr& err, int i,
                                               double dt) {
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  // After a device-detected error the updates stop: the state stays as the
+  // failing call left it (the reference throws there), so that the error's
+  // entries can be recomputed from it.
+  if (*err.word != ~0ull) return cfl_v;
   {
     const D4 x = a.in[i];
     double em = x.a, eE = x.b, ex = x.c, ey = x.d;

@@ -369,8 +369,97 @@ hipError_t fill_words(rg_ctx* c, int first, int n, unsigned long long v) {
   });
 }
 
-// Read back the error word; returns RG_OK or RG_E_DEVICE and fills c->last.
-int collect_error(rg_ctx* c) {
+// Where a call's device-detected error can be traced back to: the cells
+// each flux stage read (phases 1-2 and 4-5) and the buffers the stage
+// updates wrote (phases 3 and 6).  Null: not available.
+struct ErrSrc {
+  const D4* cells[2];
+  const D4* ext[2];
+};
+
+void add_entry(rg_error& e, const char* name, double v) {
+  if (e.n_entries >= 16) return;
+  rg_error_entry& x = e.entries[e.n_entries++];
+  std::snprintf(x.name, sizeof(x.name), "%s", name);
+  x.value = v;
+}
+
+template <class T>
+hipError_t read_one(rg_ctx* c, const T* src, size_t i, T* out) {
+  return hipMemcpyAsync(out, src + i, sizeof(T), hipMemcpyDeviceToHost, c->stream);
+}
+
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out);
+
+// The UniversalError entries of error e (see rg_error), recomputed from the
+// inputs the failing call read.
+hipError_t fill_entries(rg_ctx* c, rg_error& e, const ErrSrc& src) {
+  const int i = e.index;
+  const DMesh& m = c->dm;
+  if (e.code == RG_ERR_SINGULAR_MATRIX || e.code == RG_ERR_TOO_MANY_NEIGHBORS) {
+    if (i < 0 || i >= c->N) return hipSuccess;
+    double x = 0, y = 0, vol = 0;
+    CellGeo g{};
+    hipError_t r = read_one(c, m.px, i, &x);
+    if (r == hipSuccess) r = read_one(c, m.py, i, &y);
+    if (r == hipSuccess) r = read_one(c, m.tvol, i, &vol);
+    if (r == hipSuccess) r = read_one(c, m.cgeo, i, &g);
+    if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+    if (r != hipSuccess) return r;
+    add_entry(e, "Cell x cor", x);
+    add_entry(e, "Cell y cor", y);
+    if (e.code == RG_ERR_SINGULAR_MATRIX) {
+      add_entry(e, "Cell volume", vol);
+      add_entry(e, "Det was", g.c0);
+    }
+    return hipSuccess;
+  }
+  const int stage = e.phase >= 4 ? 1 : 0;
+  if (e.code == RG_ERR_NEGATIVE_THERMAL) {
+    const D4* x = src.ext[stage];
+    if (!x || i < 0 || i >= c->N) return hipSuccess;
+    D4 v{};
+    hipError_t r = read_one(c, x, i, &v);
+    if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+    if (r != hipSuccess) return r;
+    add_entry(e, "Cell index", static_cast<double>(i));
+    add_entry(e, "Cell mass", v.a);
+    add_entry(e, "Cell x momentum", v.c);
+    add_entry(e, "Cell y momentum", v.d);
+    add_entry(e, "Cell energy", v.b);
+    return hipSuccess;
+  }
+  if (e.code == RG_ERR_INVALID_WAVE_SPEEDS) {
+    const D4* q = src.cells[stage];
+    if (!q || i < 0 || i >= c->F) return hipSuccess;
+    hipError_t r = hipSuccess;
+    if (c->prm.order == 2) r = run_slopes(c, q, 0, 1, c->rec);
+    if (r != hipSuccess) return r;
+    k_face_entries<<<1, 1, 0, c->stream>>>(m, q, c->rec, c->prm.order, c->prm.gamma, i,
+                                            c->staging);
+    r = hipGetLastError();
+    double v[16];
+    if (r == hipSuccess)
+      r = hipMemcpyAsync(v, c->staging, sizeof(v), hipMemcpyDeviceToHost, c->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+    if (r != hipSuccess) return r;
+    static const char* const names[16] = {
+        "left density",     "left pressure",     "left x velocity",  "left y velocity",
+        "left sound speed", "left energy",       "right density",    "right pressure",
+        "right x velocity", "right y velocity",  "right sound speed", "right energy",
+        "interface velocity", "left wave speed", "center wave speed", "right wave speed"};
+    // hllc.cpp:54-70 adds them in this order: left (density, pressure, x and
+    // y velocity, sound speed, energy), right, the interface velocity, then
+    // the three wave speeds
+    for (int k = 0; k < 16; ++k) add_entry(e, names[k], v[k]);
+  }
+  return hipSuccess;
+}
+
+// Read back the error word; returns RG_OK or RG_E_DEVICE and fills c->last
+// (with the reference's entries where `src` locates the failing call's
+// inputs).
+int collect_error(rg_ctx* c, const ErrSrc* src = nullptr) {
   HIPCHK(c, hipMemcpyAsync(c->host_words, c->words, 3 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -385,6 +474,10 @@ int collect_error(rg_ctx* c) {
   e.phase = static_cast<int32_t>((w >> 36) & 0x7);
   e.step = static_cast<int32_t>(w >> 39);
   std::snprintf(e.message, sizeof(e.message), "%s", err_message(e.code));
+  if (src) {
+    const hipError_t r = fill_entries(c, e, *src);
+    if (r != hipSuccess) e.n_entries = 0;
+  }
   c->last = e;
   (void)fill_words(c, 2, 1, ~0ull);  // clear for the next call
   return RG_E_DEVICE;
@@ -394,7 +487,7 @@ bool ready(const rg_ctx* c, bool need_cells, bool need_ext) {
   return c && c->has_mesh && (!need_cells || c->has_cells) && (!need_ext || c->has_ext);
 }
 
-hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out = nullptr) {
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out) {
   Rec* r = out ? out : c->rec;
   return launch(c, KID_SLOPES, [&] {
     k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, r, dparams(c),
@@ -758,6 +851,13 @@ hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record) {
     }
   }
   return e;
+}
+
+// The buffers the FP64 step loop's stages read and write (ErrSrc).
+ErrSrc fp64_err_src(const rg_ctx* c, int integrator) {
+  if (integrator == 1) return ErrSrc{{c->prim, nullptr}, {c->ext, nullptr}};
+  const HeunBufs h = ctx_bufs(const_cast<rg_ctx*>(c));
+  return ErrSrc{{h.P, h.Pm}, {h.Xm, h.Xo}};
 }
 
 // For the duration of a rg_advance_noalloc call: the context's dt slots,
@@ -1272,7 +1372,7 @@ int rg_time_step(rg_ctx* c, double* dt) {
 int rg_reconstruct(rg_ctx* c, double* slopes, double* left, double* right) {
   if (!ready(c, true, false)) return RG_E_ARG;
   const bool pcm = c->prm.order != 2;
-  if (!pcm) HIPCHK(c, run_slopes(c, c->prim, 0, 1));
+  if (!pcm) HIPCHK(c, run_slopes(c, c->prim, 0, 1, c->rec));
   if (left || right) {
     if (!c->left) {
       HIPCHK(c, dalloc(c, &c->left, c->F));
@@ -1287,7 +1387,8 @@ int rg_reconstruct(rg_ctx* c, double* slopes, double* left, double* right) {
                                                                  c->left, c->right);
     }));
   }
-  const int rc = collect_error(c);
+  const ErrSrc src{{c->prim, nullptr}, {nullptr, nullptr}};
+  const int rc = collect_error(c, &src);
   if (rc != RG_OK) return rc;
   if (slopes && pcm) {
     std::memset(slopes, 0, sizeof(double) * 8 * static_cast<size_t>(c->N));
@@ -1319,7 +1420,8 @@ int rg_reconstruct(rg_ctx* c, double* slopes, double* left, double* right) {
 int rg_flux(rg_ctx* c, double* fm, double* fE, double* fMx, double* fMy) {
   if (!ready(c, true, false)) return RG_E_ARG;
   HIPCHK(c, run_flux(c, c->prim, 0, 1));
-  const int rc = collect_error(c);
+  const ErrSrc src{{c->prim, nullptr}, {nullptr, nullptr}};
+  const int rc = collect_error(c, &src);
   if (rc != RG_OK) return rc;
   c->has_flux = true;
   if (!fm && !fE && !fMx && !fMy) return RG_OK;
@@ -1349,7 +1451,8 @@ int rg_update_cells(rg_ctx* c) {
   a.flags = UPD_CELLS;
   c->ext_init = false;
   HIPCHK(c, run_update(c, a, 0, 3));
-  const int rc = collect_error(c);
+  const ErrSrc src{{nullptr, nullptr}, {c->ext, nullptr}};
+  const int rc = collect_error(c, &src);
   if (rc == RG_OK) c->has_cells = true;
   return rc;
 }
@@ -1399,7 +1502,8 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
-  return collect_error(c);
+  const ErrSrc src = fp64_err_src(c, integrator);
+  return collect_error(c, &src);
 }
 
 int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
@@ -1428,7 +1532,8 @@ int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
-  return collect_error(c);
+  const ErrSrc src = fp64_err_src(c, integrator);
+  return collect_error(c, &src);
 }
 
 int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out) {

@@ -26,9 +26,14 @@ class MeshS(ctypes.Structure):
             "cm_y", "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig")]
 
 
+class ErrorEntry(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("value", ctypes.c_double)]
+
+
 class Error(ctypes.Structure):
     _fields_ = [("code", ctypes.c_int32), ("index", ctypes.c_int32), ("step", ctypes.c_int32),
-                ("phase", ctypes.c_int32), ("message", ctypes.c_char * 96)]
+                ("phase", ctypes.c_int32), ("message", ctypes.c_char * 96),
+                ("n_entries", ctypes.c_int32), ("entries", ErrorEntry * 16)]
 
 
 class RichGPUError(RuntimeError):
@@ -41,6 +46,9 @@ class RichGPUError(RuntimeError):
         self.step = err.step
         self.phase = err.phase
         self.text = err.message.decode(errors="replace")
+        # the reference's UniversalError entries, in its order
+        self.entries = [(err.entries[k].name.decode(errors="replace"), err.entries[k].value)
+                        for k in range(max(0, min(err.n_entries, 16)))]
         super().__init__(f"{self.text} (code {self.code}, index {self.index}, step {self.step}, "
                          f"phase {self.phase}, status {status})")
 

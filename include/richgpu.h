@@ -94,12 +94,34 @@ typedef struct rg_mesh {
   const int32_t *ghost_orig;               /* [G] GetOriginalIndex           */
 } rg_mesh;
 
+/* One UniversalError::AddEntry(name, value) of the reference. */
+typedef struct rg_error_entry {
+  char name[32];
+  double value;
+} rg_error_entry;
+
 typedef struct rg_error {
   int32_t code;     /* RG_ERR_* or 0                                  */
   int32_t index;    /* cell or face index of the first failure        */
   int32_t step;     /* step of rg_advance (0 outside it)              */
   int32_t phase;    /* 0 cfl, 1 slopes, 2 fluxes, 3 cells (+3 stage 2) */
   char message[96]; /* reference message text                          */
+  /* The entries the reference's exception carries, in its order:
+   *   RG_ERR_NEGATIVE_THERMAL     "Cell index", "Cell mass", "Cell x momentum",
+   *                               "Cell y momentum", "Cell energy"
+   *                               (simple_cell_updater.cpp:50-57)
+   *   RG_ERR_INVALID_WAVE_SPEEDS  "left density" .. "right wave speed", 16
+   *                               entries (hllc.cpp:46-71)
+   *   RG_ERR_SINGULAR_MATRIX      "Cell x cor", "Cell y cor", "Cell volume",
+   *                               "Det was" (LinearGaussImproved.cpp:111-118)
+   *   RG_ERR_TOO_MANY_NEIGHBORS   "Cell x cor", "Cell y cor" (:73-79)
+   * none for the other codes (the reference adds none).  Filled by the
+   * plugin entry points and by rg_advance / rg_advance_noalloc on an RG_FP64
+   * context without instances (n_entries = 0 elsewhere): after a
+   * device-detected error the step's updates stop, so the state the failing
+   * call read is still there to recompute them from. */
+  int32_t n_entries;
+  rg_error_entry entries[16];
 } rg_error;
 
 /* lifecycle ---------------------------------------------------------- */
