@@ -8,10 +8,19 @@
 //  3. runs the same step through the device-resident loop rg_advance on a
 //     second context and checks that both give the same cells and
 //     extensives, bit for bit;
-//  4. times second-order Heun steps with rg_advance.
+//  4. times second-order Heun steps with rg_advance;
+//  5. with an output file, runs K hdsim::TimeAdvance2Heun steps
+//     (hdsim2d.cpp:323-449) in the call order of the plugin adapters -- the
+//     time step, the flux calculator, ZeroForce and the extensive updater on
+//     host copies of the mid extensives, the cell updater, the second stage,
+//     the host average of old and mid extensives, the cell updater -- with
+//     every plugin a C-ABI call and every state a host array in between, and
+//     writes the initial cells, the dts, the cells and the extensives there
+//     (raw doubles: cells [4][N], K dts, cells [4][N], extensives [4][N])
+//     for a checker to compare.
 // Prints one JSON line; exits non-zero on any error or mismatch.
 //
-//   kh_driver [nx ny heun_steps]      (default 512 256 100)
+//   kh_driver [nx ny heun_steps [K out_file]]      (default 512 256 100)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,7 +52,9 @@ int main(int argc, char** argv) {
   const int nx = argc > 1 ? std::atoi(argv[1]) : 512;
   const int ny = argc > 2 ? std::atoi(argv[2]) : 256;
   const int steps = argc > 3 ? std::atoi(argv[3]) : 100;
-  if (nx < 4 || ny < 4 || steps < 1) return 2;
+  const int plugin_steps = argc > 5 ? std::atoi(argv[4]) : 0;
+  const char* out_path = argc > 5 ? argv[5] : nullptr;
+  if (nx < 4 || ny < 4 || steps < 1 || plugin_steps < 0) return 2;
 
   // 1. the mesh: periodic [0,2]x[0,1], jitter 1e-3 of the spacing
   const rm_lattice_params lp{nx, ny, 0.0, 2.0, 0.0, 1.0, 1e-3, 1, RM_BC_PERIODIC, 0};
@@ -124,6 +135,93 @@ int main(int argc, char** argv) {
   for (int i = 0; i < N; ++i) mass += xb[i];
   rg_destroy(a);
   rg_destroy(b);
+
+  // 5. TimeAdvance2Heun in the adapters' call order, host arrays in between
+  if (out_path && plugin_steps > 0) {
+    rg_ctx* h = nullptr;
+    CHECK(nullptr, rg_create(&prm, &h));
+    CHECK(h, rg_set_mesh(h, &mesh));
+    std::vector<double> cells(4 * static_cast<size_t>(N)), ext(4 * static_cast<size_t>(N)),
+        mid(4 * static_cast<size_t>(N)), midc(4 * static_cast<size_t>(N)),
+        flux(4 * static_cast<size_t>(F)), dts;
+    auto row = [](std::vector<double>& v, int k, int n) { return v.data() + static_cast<size_t>(k) * n; };
+    std::memcpy(row(cells, 0, N), d.data(), sizeof(double) * N);
+    std::memcpy(row(cells, 1, N), p.data(), sizeof(double) * N);
+    std::memcpy(row(cells, 2, N), vx.data(), sizeof(double) * N);
+    std::memcpy(row(cells, 3, N), vy.data(), sizeof(double) * N);
+    const std::vector<double> cells0 = cells;
+    // hdsim's constructor: extensives from the cells (hdsim2d.cpp:41-66)
+    CHECK(h, rg_set_cells(h, row(cells, 0, N), row(cells, 1, N), row(cells, 2, N), row(cells, 3, N)));
+    CHECK(h, rg_init_extensives(h));
+    CHECK(h, rg_get_extensives(h, row(ext, 0, N), row(ext, 1, N), row(ext, 2, N), row(ext, 3, N)));
+    auto set_cells = [&](std::vector<double>& c) {
+      return rg_set_cells(h, row(c, 0, N), row(c, 1, N), row(c, 2, N), row(c, 3, N));
+    };
+    auto zero_force = [&](std::vector<double>& x, double dt_) {  // zero_force.cpp:3-18
+      for (int i = 0; i < N; ++i) {
+        row(x, 0, N)[i] += dt_ * 0.0;
+        row(x, 2, N)[i] += 0.0 * dt_;
+        row(x, 3, N)[i] += 0.0 * dt_;
+        row(x, 1, N)[i] += dt_ * 0.0;
+      }
+    };
+    // eu_: fluxes and extensives in, extensives out
+    auto update_extensives = [&](std::vector<double>& x, double dt_) {
+      int rc = rg_set_extensives(h, row(x, 0, N), row(x, 1, N), row(x, 2, N), row(x, 3, N));
+      if (rc == RG_OK)
+        rc = rg_set_fluxes(h, row(flux, 0, F), row(flux, 1, F), row(flux, 2, F), row(flux, 3, F));
+      if (rc == RG_OK) rc = rg_update_extensives(h, dt_);
+      if (rc == RG_OK)
+        rc = rg_get_extensives(h, row(x, 0, N), row(x, 1, N), row(x, 2, N), row(x, 3, N));
+      return rc;
+    };
+    // cu_: extensives in, cells out
+    auto update_cells = [&](std::vector<double>& x, std::vector<double>& c) {
+      int rc = rg_set_extensives(h, row(x, 0, N), row(x, 1, N), row(x, 2, N), row(x, 3, N));
+      if (rc == RG_OK) rc = rg_update_cells(h);
+      if (rc == RG_OK) rc = rg_get_cells(h, row(c, 0, N), row(c, 1, N), row(c, 2, N), row(c, 3, N));
+      return rc;
+    };
+    auto flux_calc = [&](std::vector<double>& c) {
+      int rc = set_cells(c);
+      if (rc == RG_OK) rc = rg_set_face_velocity(h, nullptr, nullptr);  // static mesh
+      if (rc == RG_OK)
+        rc = rg_flux(h, row(flux, 0, F), row(flux, 1, F), row(flux, 2, F), row(flux, 3, F));
+      return rc;
+    };
+    for (int s = 0; s < plugin_steps; ++s) {
+      double dt_h = 0;
+      CHECK(h, set_cells(cells));
+      CHECK(h, rg_time_step(h, &dt_h));  // tsf_, twice around ApplyFix (no motion here)
+      CHECK(h, rg_time_step(h, &dt_h));
+      CHECK(h, flux_calc(cells));        // mid_fluxes
+      mid = ext;
+      zero_force(mid, dt_h);             // ExternalForceContribution
+      CHECK(h, update_extensives(mid, dt_h));
+      CHECK(h, update_cells(mid, midc));  // mid_cells on the (unchanged) mesh
+      CHECK(h, flux_calc(midc));          // fluxes of the mid cells
+      zero_force(mid, dt_h);
+      CHECK(h, update_extensives(mid, dt_h));
+      for (int i = 0; i < N; ++i) {       // average_extensive (hdsim2d.cpp:311-320)
+        row(ext, 0, N)[i] = 0.5 * (row(mid, 0, N)[i] + row(ext, 0, N)[i]);
+        row(ext, 1, N)[i] = 0.5 * (row(mid, 1, N)[i] + row(ext, 1, N)[i]);
+        row(ext, 2, N)[i] = (row(mid, 2, N)[i] + row(ext, 2, N)[i]) * 0.5;
+        row(ext, 3, N)[i] = (row(mid, 3, N)[i] + row(ext, 3, N)[i]) * 0.5;
+      }
+      CHECK(h, update_cells(ext, cells));
+      dts.push_back(dt_h);
+    }
+    rg_destroy(h);
+    FILE* fo = std::fopen(out_path, "wb");
+    if (!fo) return fail(nullptr, "fopen", -1);
+    const bool ok = std::fwrite(cells0.data(), sizeof(double), cells0.size(), fo) == cells0.size() &&
+                    std::fwrite(dts.data(), sizeof(double), dts.size(), fo) == dts.size() &&
+                    std::fwrite(cells.data(), sizeof(double), cells.size(), fo) == cells.size() &&
+                    std::fwrite(ext.data(), sizeof(double), ext.size(), fo) == ext.size();
+    std::fclose(fo);
+    if (!ok) return fail(nullptr, "fwrite", -1);
+  }
+
   std::printf(
       "{\"cells\": %d, \"faces\": %d, \"plugin_step_equals_device_step\": %s, \"dt\": %.17g, "
       "\"heun_steps\": %d, \"ms_per_step\": %.4f, \"mcell_updates_per_s\": %.1f, "
