@@ -247,7 +247,13 @@ def adjoint_leg(args):
     """Sedov inverse problem (static rigid box): timings of the checkpointed
     forward and of the adjoint, and the adjoint's directional derivatives
     against central differences (h = 1e-6, directions relative to the
-    parameters) and optionally against the forward-mode CPU oracle."""
+    parameters; each difference taken cell by cell before the loss) and
+    optionally against the forward-mode CPU oracle.  For the first
+    --adjoint-fp-dirs directions and the blast energy alone, the oracle's
+    branch fingerprints of the runs at -h and +h (bit-identical to the GPU
+    runs) record which steps switch a branch between them: a jump (limiter
+    gate or guard, shock switch) or a kink (extremum neighbour, active face,
+    HLLC wave pattern, CFL argmin)."""
     import numpy as np
     import torch
     from rich_amd.inverse import SedovInverse
@@ -284,24 +290,66 @@ def adjoint_leg(args):
     rng = np.random.default_rng(0)
     dirs = [(torch.from_numpy(rng.standard_normal(P.N)), float(rng.standard_normal()))
             for _ in range(args.adjoint_dirs)]
+    e_only = (torch.zeros(P.N, dtype=torch.float64), 1.0)
     h = 1e-6
+
+    def final_cells(xi, xe, sign):
+        with torch.no_grad():
+            return P.forward(u0 + sign * h * xi * u0, E + sign * h * xe * E)[1].numpy()
+
+    # each direction's runs at -h and +h serve both rings
+    pairs = [(final_cells(xi, xe, -1.0), final_cells(xi, xe, 1.0)) for xi, xe in dirs + [e_only]]
+    log("adjoint: finite-difference runs done")
+    # branch fingerprints of the oracle runs at -h, 0, +h
+    fp = None
+    kfp = min(args.adjoint_fp_dirs, len(dirs))
+    if kfp > 0:
+        import pyoracle as po
+        threads = min(16, os.cpu_count() or 1)
+        o = po.Oracle(P.mesh, order=2, nthreads=threads)
+
+        def fprint(xi, xe, sign):
+            c = P.cells(u0 + sign * h * xi * u0, E + sign * h * xe * E).numpy()
+            return o.run(c, o.init_extensives(c), steps, fingerprint=True)[3]
+
+        t = time.perf_counter()
+        base = fprint(e_only[0], 0.0, 0.0)
+        fp = []
+        for xi, xe in dirs[:kfp] + [e_only]:
+            diff = (fprint(xi, xe, -1.0) != base) | (fprint(xi, xe, 1.0) != base)
+            fp.append({"steps_with_jumps": int(diff[:, 6:8].any(axis=1).sum()),
+                       "steps_with_kinks": int(diff[:, 2:6].any(axis=1).sum())})
+        log(f"adjoint: branch fingerprints of {kfp} directions and E done")
+        fp_meta = {"directions": kfp, "threads": threads, "seconds": time.perf_counter() - t}
     for key, r_ring in (("ring_0.30_as_specified", 0.3), ("ring_0.22_at_the_shock", 0.22)):
         Q = P if r_ring == 0.3 else SedovInverse(n=n, steps=steps, every=every, r_ring=r_ring)
         L, gu, gE = Q.gradient(u0, E)
+        ring, target = Q.ring.numpy(), Q.target.numpy()
+
+        def fd_of(pair):
+            # (L(+h) - L(-h)) / 2h with L = sum (rho - target)^2, factored
+            # cell by cell so that two nearly equal losses never cancel
+            a, b = pair[0][0, ring], pair[1][0, ring]
+            return float(np.sum((b - a) * (a + b - 2.0 * target))) / (2 * h)
+
         errs, an_all = [], []
-        for xi, xe in dirs:
-            du, dE = xi * u0, xe * E
-            an = float((gu * du).sum()) + gE * float(dE)
-            fd = (Q.loss_value(u0 + h * du, E + h * dE) -
-                  Q.loss_value(u0 - h * du, E - h * dE)) / (2 * h)
-            errs.append(abs(fd - an) / max(abs(an), 1e-300))
+        for (xi, xe), pair in zip(dirs, pairs):
+            an = float((gu * xi * u0).sum()) + gE * float(xe * E)
+            errs.append(abs(fd_of(pair) - an) / max(abs(an), 1e-300))
             an_all.append(an)
-        fdE = (Q.loss_value(u0, E + h) - Q.loss_value(u0, E - h)) / (2 * h)
+        fdE = fd_of(pairs[-1])
         rec = {"loss": L, "dL_dE": gE, "grad_u_norm": float(gu.norm()), "h": h,
                "fd_directions": len(errs), "fd_rel_err_median": float(np.median(errs)),
                "fd_rel_err_max": float(np.max(errs)),
                "fd_within_1e-6": int(np.sum(np.array(errs) <= 1e-6)),
                "E_only_fd": fdE, "E_only_fd_rel_err": abs(fdE - gE) / max(abs(gE), 1e-300)}
+        if fp is not None:
+            rec["branch_switches"] = {
+                "directions": [dict(fp[k], fd_rel_err=errs[k]) for k in range(kfp)],
+                "E_only": dict(fp[-1], fd_rel_err=rec["E_only_fd_rel_err"]),
+                "directions_without_switch": int(sum(
+                    1 for d in fp[:kfp] if d["steps_with_jumps"] == 0 and d["steps_with_kinks"] == 0)),
+                **fp_meta}
         log(f"adjoint: {key} finite differences done")
         if args.adjoint_cpu_ref > 0:
             log(f"adjoint: {key} forward-mode CPU reference")
@@ -315,8 +363,7 @@ def adjoint_leg(args):
                 dc[k, 1] = (Q.gamma - 1.0) * (xi * u0 + xe * Q.w).numpy()
             t = time.perf_counter()
             cT, dcT, _, _, _ = o.run_tangent(c0, dc, steps)
-            ring = Q.ring.numpy()
-            resid = cT[0, ring] - Q.target.numpy()
+            resid = cT[0, ring] - target
             ref = [float(np.sum(2.0 * resid * dcT[k, 0, ring])) for k in range(K)]
             rel = [abs(an_all[k] - ref[k]) / max(abs(ref[k]), 1e-300) for k in range(K)]
             rec["cpu_reference"] = {"directions": K, "rel_err_max": float(np.max(rel)),
@@ -351,6 +398,8 @@ def main():
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--adjoint-dirs", type=int, default=32,
                     help="seeded random directions of the finite-difference check")
+    ap.add_argument("--adjoint-fp-dirs", type=int, default=1, metavar="K",
+                    help="directions (and E alone) whose -h/+h oracle runs are fingerprinted")
     ap.add_argument("--adjoint-cpu-ref", type=int, default=8, metavar="K",
                     help="also compare K directional derivatives with the forward-mode CPU "
                          "oracle (dual numbers, OpenMP, 8 per pass); 0 = skip")

@@ -75,6 +75,8 @@ def lib():
         L.ro_lgi.argtypes = [P(Mesh_), P(Params_), Quad_, ctypes.c_void_p, ctypes.c_void_p,
                              Quad_, Quad_]
         L.ro_pcm.argtypes = [P(Mesh_), P(Params_), Quad_, Quad_, Quad_]
+        L.ro_fingerprint.restype = None
+        L.ro_fingerprint.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         for n in ("ro_point_velocity", "ro_point_fix", "ro_face_velocity"):
             getattr(L, n).restype = None
         L.ro_point_velocity.argtypes = [P(Mesh_), ctypes.c_double] + [ctypes.c_void_p] * 4 + [
@@ -246,17 +248,27 @@ class Oracle:
                                  _quad(np.ascontiguousarray(cells[:, :self.N])), _quad(ext))
         return ext
 
-    def run(self, cells, ext, nsteps, integrator=2, w=None):
-        """hdsim TimeAdvance (1) / TimeAdvance2Heun (2); returns (cells, ext, dts)."""
+    def run(self, cells, ext, nsteps, integrator=2, w=None, fingerprint=False):
+        """hdsim TimeAdvance (1) / TimeAdvance2Heun (2); returns (cells, ext, dts),
+        and with `fingerprint` also the branch fingerprints (nsteps, 8) uint64:
+        per step, one sum per kind of branch (see ro_fingerprint)."""
         c = self._cells_ng(cells)
         x = np.array(ext, dtype=np.float64, order="C")
         dts = np.zeros(max(nsteps, 1))
         wx, wy = (None, None) if w is None else (np.ascontiguousarray(w[0]),
                                                  np.ascontiguousarray(w[1]))
-        e = lib().ro_run(ctypes.byref(self.m), ctypes.byref(self.prm), integrator, nsteps,
-                         _p(wx), _p(wy), _quad(c), _quad(x), _p(dts))
+        fp = np.zeros((max(nsteps, 1), 8), dtype=np.uint64) if fingerprint else None
+        if fingerprint:
+            lib().ro_fingerprint(_p(fp), nsteps)
+        try:
+            e = lib().ro_run(ctypes.byref(self.m), ctypes.byref(self.prm), integrator, nsteps,
+                             _p(wx), _p(wy), _quad(c), _quad(x), _p(dts))
+        finally:
+            if fingerprint:
+                lib().ro_fingerprint(None, 0)
         self._check(e)
-        return c[:, :self.N].copy(), x, dts[:nsteps]
+        out = (c[:, :self.N].copy(), x, dts[:nsteps])
+        return out + (fp[:nsteps],) if fingerprint else out
 
     def run_tangent(self, cells, dcells, nsteps, integrator=2, w=None):
         """Forward-mode derivatives of init_extensives + run: cells (4, N)

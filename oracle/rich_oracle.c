@@ -12,6 +12,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -43,6 +46,58 @@ static real smin(real a, real b) { return (b < a) ? b : a; }
 static real smax(real a, real b) { return (a < b) ? b : a; }
 /* tessellation/geometry.cpp:18-21  abs(Vector2D) */
 static double vabs(double x, double y) { return sqrt(x * x + y * y); }
+
+/* ----------------------------------------------------- fingerprint -- */
+/* Optional record of every branch the step takes (shock switch, limiter
+ * gates, guards and active faces, neighbour extrema, HLLC wave pattern and
+ * side, CFL argmin), summed per step as 64-bit hashes: two runs took the
+ * same branches in a step exactly when (barring hash collisions) their sums
+ * agree.  Off unless ro_fingerprint() gave a buffer.  Test infrastructure:
+ * it tells where central differences may straddle a discontinuity. */
+static uint64_t* g_fp = NULL;
+static int32_t g_fp_n = 0, g_fp_step = -1, g_fp_stage = 0;
+static uint64_t mix64(uint64_t x) { /* splitmix64 finaliser */
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+static uint64_t fp_fold(uint64_t h, uint64_t v) { return mix64(h ^ (v + 0x9e3779b97f4a7c15ull)); }
+/* Kinds of branch.  Jumps (the value is discontinuous across the switch):
+ * 6 the shock switch, 7 the limiters' gates and guards.  Kinks (continuous
+ * value, derivative discontinuous): 2 HLLC's wave pattern and the sides
+ * giving its outer wave speeds, 3 the CFL argmin, 4 and 5 the faces that
+ * set psi, the clamp at 0 and the neighbours giving the extrema in
+ * slope_limit and shocked_slope_limit. */
+enum { RO_FP_KINDS = 8, RO_FP_THREADS = 256 };
+/* per-thread partial sums of the current step, one 64-byte line each;
+ * sums commute, so the record does not depend on the thread count */
+static uint64_t g_fp_part[RO_FP_THREADS][RO_FP_KINDS];
+static int fp_thread(void) {
+#ifdef _OPENMP
+  return omp_get_thread_num() % RO_FP_THREADS;
+#else
+  return 0;
+#endif
+}
+static void fp_add(uint64_t kind, uint64_t item, uint64_t code) {
+  if (!g_fp || g_fp_step < 0 || g_fp_step >= g_fp_n) return;
+  const uint64_t h = fp_fold(fp_fold(fp_fold(kind, (uint64_t)g_fp_stage), item), code);
+  /* atomic, though each thread has its own line (more than RO_FP_THREADS
+   * threads would share rows) */
+#pragma omp atomic
+  g_fp_part[fp_thread()][kind] += h;
+}
+/* fold the partial sums into the step's record */
+static void fp_flush(void) {
+  if (!g_fp || g_fp_step < 0 || g_fp_step >= g_fp_n) return;
+  for (int t = 0; t < RO_FP_THREADS; ++t)
+    for (int k = 0; k < RO_FP_KINDS; ++k) {
+      g_fp[(size_t)g_fp_step * RO_FP_KINDS + k] += g_fp_part[t][k];
+      g_fp_part[t][k] = 0;
+    }
+}
 
 /* ---------------------------------------------------------------- EOS -- */
 /* newtonian/common/ideal_gas.cpp:13-16 */
@@ -120,7 +175,8 @@ static cons_t add_scaled_diff(const cons_t* f, real s, const cons_t* us, const c
   return r;
 }
 /* hllc.cpp:97-154 (massflux_ = true, the Hllc() default, hllc.hpp:22) */
-static int hllc(const prim_t* left, const prim_t* right, double velocity, cons_t* out) {
+static int hllc(const prim_t* left, const prim_t* right, double velocity, cons_t* out,
+                int* branch) {
   if (is_nan_ref(right->vx)) return RO_HLLC_NAN;
   prim_t l = *left, r = *right;
   /* Velocity -= velocity*normaldir, normaldir = (1,0) */
@@ -137,16 +193,19 @@ static int hllc(const prim_t* left, const prim_t* right, double velocity, cons_t
   const cons_t usl = starred_state(&l, sl, ss);
   const cons_t usr = starred_state(&r, sr, ss);
   cons_t f;
+  int b;
   if (sl > 0)
-    f = fl;
+    f = fl, b = 0;
   else if (sl <= 0 && ss >= 0)
-    f = add_scaled_diff(&fl, sl, &usl, &ul);
+    f = add_scaled_diff(&fl, sl, &usl, &ul), b = 1;
   else if (ss < 0 && sr >= 0)
-    f = add_scaled_diff(&fr, sr, &usr, &ur);
+    f = add_scaled_diff(&fr, sr, &usr, &ur), b = 2;
   else if (sr < 0)
-    f = fr;
+    f = fr, b = 3;
   else
     return RO_INVALID_WAVE_SPEEDS;
+  if (branch) /* the pattern, and which side gave each outer wave speed */
+    *branch = b | ((r.vx - r.c < l.vx - l.c) << 2) | ((l.vx + l.c < r.vx + r.c) << 3);
   /* back to the lab frame */
   f.E = f.E + (f.mx * (1.0 * velocity) + f.my * (0.0 * velocity) + 0.5 * f.m * velocity * velocity);
   const real vm = velocity * f.m;
@@ -161,7 +220,7 @@ int ro_hllc(const double L[6], const double R[6], double velocity, double out[4]
   const prim_t l = {L[0], L[1], L[2], L[3], L[4], L[5]};
   const prim_t r = {R[0], R[1], R[2], R[3], R[4], R[5]};
   cons_t f;
-  const int rc = hllc(&l, &r, velocity, &f);
+  const int rc = hllc(&l, &r, velocity, &f, NULL);
   if (rc == RO_OK) {
     out[0] = f.m;
     out[1] = f.mx;
@@ -201,11 +260,11 @@ static cons_t rotate_back(const cons_t* c, double nx, double ny, double px, doub
 }
 /* simple_flux_calculator.cpp:153-165 */
 static int rotate_solve_rotate_back(const prim_t* l, const prim_t* r, double velocity, double nx,
-                                    double ny, double px, double py, cons_t* out) {
+                                    double ny, double px, double py, cons_t* out, int* branch) {
   const prim_t rl = rotate(l, nx, ny, px, py);
   const prim_t rr = rotate(r, nx, ny, px, py);
   cons_t f;
-  const int rc = hllc(&rl, &rr, velocity, &f);
+  const int rc = hllc(&rl, &rr, velocity, &f, branch);
   if (rc != RO_OK) return rc;
   *out = rotate_back(&f, nx, ny, px, py);
   return RO_OK;
@@ -406,16 +465,23 @@ static real pressure_ratio(const cc_t* cell, int n, const cc_t* ncell) {
 
 /* one field of slope_limit (LinearGaussImproved.cpp:232-267); absguard
  * selects the velocity form of the +-1e-9 guard */
-static void limit_field(real dphi, real cval, real cell, real maxd, real mind,
-                        int absguard, real* psi) {
+static int limit_field(real dphi, real cval, real cell, real maxd, real mind,
+                       int absguard, real* psi) {
+  /* returned: the branches taken (fingerprint) */
+  int code = 0;
   if (fabs(dphi) > 0.05 * smax(fabs(maxd), fabs(mind)) || cval * cell < 0) {
     const real hi = absguard ? fabs(1e-9 * cell) : 1e-9 * cell;
     const real lo = absguard ? -fabs(1e-9 * cell) : -1e-9 * cell;
-    if (dphi > hi)
+    code = 1 | ((fabs(maxd) < fabs(mind)) << 4);
+    if (dphi > hi) {
+      code |= 2 | ((maxd / dphi < *psi) << 3);
       *psi = smin(*psi, maxd / dphi);
-    else if (dphi < lo)
+    } else if (dphi < lo) {
+      code |= 4 | ((mind / dphi < *psi) << 3);
       *psi = smin(*psi, mind / dphi);
+    }
   }
+  return code;
 }
 
 /* LinearGaussImproved.cpp:174-300 */
@@ -423,10 +489,16 @@ static void slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, int n,
                         const int32_t* faces, const int32_t* nbr, const cc_t* ncell,
                         slope_t* sl) {
   cc_t cmax = *cell, cmin = *cell;
+  uint64_t h = 0, hj = 0; /* fingerprint: kinks (extrema, active faces), jumps */
   for (int i = 0; i < n; ++i) {
     const int32_t f = faces[i];
     if (orig_index(m, m->face_first[f]) == orig_index(m, m->face_second[f])) continue;
     const cc_t* t = &ncell[i];
+    if (g_fp)
+      h = fp_fold(h, (uint64_t)((cmax.d < t->d) | (cmax.p < t->p) << 1 | (cmax.vx < t->vx) << 2 |
+                                (cmax.vy < t->vy) << 3 | (t->d < cmin.d) << 4 |
+                                (t->p < cmin.p) << 5 | (t->vx < cmin.vx) << 6 |
+                                (t->vy < cmin.vy) << 7 | i << 8));
     cmax.d = smax(cmax.d, t->d);
     cmax.p = smax(cmax.p, t->p);
     cmax.vx = smax(cmax.vx, t->vx);
@@ -446,11 +518,17 @@ static void slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, int n,
     face_centroid(m, faces[i], &fx, &fy);
     const cc_t cv = interp(cell, sl, fx, fy, cmx, cmy);
     const cc_t dphi = {cv.d - cell->d, cv.p - cell->p, cv.vx - cell->vx, cv.vy - cell->vy};
-    limit_field(dphi.d, cv.d, cell->d, maxd.d, mind.d, 0, &psi[0]);
-    limit_field(dphi.p, cv.p, cell->p, maxd.p, mind.p, 0, &psi[1]);
-    limit_field(dphi.vx, cv.vx, cell->vx, maxd.vx, mind.vx, 1, &psi[2]);
-    limit_field(dphi.vy, cv.vy, cell->vy, maxd.vy, mind.vy, 1, &psi[3]);
+    const int c0 = limit_field(dphi.d, cv.d, cell->d, maxd.d, mind.d, 0, &psi[0]);
+    const int c1 = limit_field(dphi.p, cv.p, cell->p, maxd.p, mind.p, 0, &psi[1]);
+    const int c2 = limit_field(dphi.vx, cv.vx, cell->vx, maxd.vx, mind.vx, 1, &psi[2]);
+    const int c3 = limit_field(dphi.vy, cv.vy, cell->vy, maxd.vy, mind.vy, 1, &psi[3]);
+    if (g_fp) { /* bits 0-2 gate and guard (jumps), 3-4 the rest (kinks) */
+      hj = fp_fold(hj, (uint64_t)((c0 & 7) | (c1 & 7) << 3 | (c2 & 7) << 6 | (c3 & 7) << 9 | i << 12));
+      h = fp_fold(h, (uint64_t)((c0 >> 3) | (c1 >> 3) << 2 | (c2 >> 3) << 4 | (c3 >> 3) << 6 | i << 8));
+    }
   }
+  fp_add(4, (uint64_t)ci, h);
+  fp_add(7, (uint64_t)ci, hj);
   sl->x.d *= psi[0];
   sl->y.d *= psi[0];
   sl->x.p *= psi[1];
@@ -463,11 +541,18 @@ static void slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, int n,
 
 /* one field of shocked_slope_limit (LinearGaussImproved.cpp:343-366); the
  * guard has no abs() for any field (:346,352,358,364) */
-static void shocked_field(real dphi, real cval, real cell, real nval, real maxd,
-                          real mind, double theta, real* psi) {
+static int shocked_field(real dphi, real cval, real cell, real nval, real maxd,
+                         real mind, double theta, real* psi) {
+  int code = 0; /* the branches taken (fingerprint) */
   if (fabs(dphi) > 0.1 * smax(fabs(maxd), fabs(mind)) || cval * cell < 0) {
-    if (fabs(dphi) > 1e-9 * cell) *psi = smin(*psi, smax(theta * (nval - cell) / dphi, 0.0));
+    code = 1 | ((fabs(maxd) < fabs(mind)) << 4);
+    if (fabs(dphi) > 1e-9 * cell) {
+      const real q = theta * (nval - cell) / dphi;
+      code |= 2 | ((q < 0.0) << 2) | ((smax(q, 0.0) < *psi) << 3);
+      *psi = smin(*psi, smax(theta * (nval - cell) / dphi, 0.0));
+    }
   }
+  return code;
 }
 
 /* LinearGaussImproved.cpp:302-399 */
@@ -475,8 +560,14 @@ static void shocked_slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, 
                                 const int32_t* faces, const cc_t* ncell, double theta,
                                 slope_t* sl) {
   cc_t cmax = *cell, cmin = *cell;
+  uint64_t h = 0, hj = 0; /* fingerprint: kinks (extrema, active faces), jumps */
   for (int i = 0; i < n; ++i) {
     const cc_t* t = &ncell[i];
+    if (g_fp)
+      h = fp_fold(h, (uint64_t)((cmax.d < t->d) | (cmax.p < t->p) << 1 | (cmax.vx < t->vx) << 2 |
+                                (cmax.vy < t->vy) << 3 | (t->d < cmin.d) << 4 |
+                                (t->p < cmin.p) << 5 | (t->vx < cmin.vx) << 6 |
+                                (t->vy < cmin.vy) << 7 | i << 8));
     cmax.d = smax(cmax.d, t->d);
     cmax.p = smax(cmax.p, t->p);
     cmax.vx = smax(cmax.vx, t->vx);
@@ -495,11 +586,19 @@ static void shocked_slope_limit(const ro_mesh* m, int32_t ci, const cc_t* cell, 
     face_centroid(m, faces[i], &fx, &fy);
     const cc_t cv = interp(cell, sl, fx, fy, cmx, cmy);
     const cc_t dphi = {cv.d - cell->d, cv.p - cell->p, cv.vx - cell->vx, cv.vy - cell->vy};
-    shocked_field(dphi.d, cv.d, cell->d, ncell[i].d, maxd.d, mind.d, theta, &psi[0]);
-    shocked_field(dphi.p, cv.p, cell->p, ncell[i].p, maxd.p, mind.p, theta, &psi[1]);
-    shocked_field(dphi.vx, cv.vx, cell->vx, ncell[i].vx, maxd.vx, mind.vx, theta, &psi[2]);
-    shocked_field(dphi.vy, cv.vy, cell->vy, ncell[i].vy, maxd.vy, mind.vy, theta, &psi[3]);
+    const int c0 = shocked_field(dphi.d, cv.d, cell->d, ncell[i].d, maxd.d, mind.d, theta, &psi[0]);
+    const int c1 = shocked_field(dphi.p, cv.p, cell->p, ncell[i].p, maxd.p, mind.p, theta, &psi[1]);
+    const int c2 =
+        shocked_field(dphi.vx, cv.vx, cell->vx, ncell[i].vx, maxd.vx, mind.vx, theta, &psi[2]);
+    const int c3 =
+        shocked_field(dphi.vy, cv.vy, cell->vy, ncell[i].vy, maxd.vy, mind.vy, theta, &psi[3]);
+    if (g_fp) { /* bits 0-1 gate and guard (jumps), 2-4 the rest (kinks) */
+      hj = fp_fold(hj, (uint64_t)((c0 & 3) | (c1 & 3) << 2 | (c2 & 3) << 4 | (c3 & 3) << 6 | i << 8));
+      h = fp_fold(h, (uint64_t)((c0 >> 2) | (c1 >> 2) << 3 | (c2 >> 2) << 6 | (c3 >> 2) << 9 | i << 12));
+    }
   }
+  fp_add(5, (uint64_t)ci, h);
+  fp_add(7, (uint64_t)ci, hj);
   sl->x.d *= psi[0];
   sl->y.d *= psi[0];
   sl->x.p *= psi[1];
@@ -547,6 +646,7 @@ static int calc_slope(const ro_mesh* m, const ro_params* prm, ro_cells cng, int3
     /* is_shock, LinearGaussImproved.cpp:150-157 */
     const int cond1 = (res->x.vx + res->y.vy) * width < (-prm->delta_v) * cs;
     const int cond2 = pressure_ratio(&cell, n, ncell) < prm->delta_p;
+    fp_add(6, (uint64_t)ci, (uint64_t)(cond1 | cond2 << 1)); /* the shock switch */
     if (!(cond1 || cond2))
       slope_limit(m, ci, &cell, n, faces, nbr, ncell, res);
     else
@@ -691,7 +791,7 @@ void ro_pcm(const ro_mesh* m, const ro_params* prm, ro_cells cng, ro_cells left,
 
 /* ------------------------------------------------------ flux calculators */
 static int modular_face_flux(const ro_mesh* m, const ro_params* prm, int32_t f, const cc_t* lf,
-                             const cc_t* rt, double wx, double wy, cons_t* out) {
+                             const cc_t* rt, double wx, double wy, cons_t* out, int* branch) {
   const int32_t N = m->n_cells;
   const int32_t a = m->face_first[f], b = m->face_second[f];
   /* calc_parallel_normal (modular_flux_calculator.cpp:12-22) */
@@ -705,7 +805,7 @@ static int modular_face_flux(const ro_mesh* m, const ro_params* prm, int32_t f, 
   const double speed = (nx * wx + ny * wy) / vabs(nx, ny);
   const prim_t pl = convert_to_primitive(lf, prm->gamma);
   const prim_t pr = convert_to_primitive(rt, prm->gamma);
-  return rotate_solve_rotate_back(&pl, &pr, speed, nx, ny, px, py, out);
+  return rotate_solve_rotate_back(&pl, &pr, speed, nx, ny, px, py, out, branch);
 }
 
 /* Reflect (geometry.cpp:117-120) applied by reflect()
@@ -724,7 +824,7 @@ static prim_t reflect_prim(const prim_t* s, double ax, double ay) {
  * (simple_flux_calculator.cpp:80-126,167-189).  Where the reference would
  * read cells[ghost] (a non-mirror ghost face) the filled ghost slot is used. */
 static int simple_face_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, int32_t f,
-                            double wx, double wy, cons_t* out) {
+                            double wx, double wy, cons_t* out, int* branch) {
   const int32_t N = m->n_cells;
   const int32_t a = m->face_first[f], b = m->face_second[f];
   const double px = m->v2x[f] - m->v1x[f], py = m->v2y[f] - m->v1y[f];
@@ -756,7 +856,7 @@ static int simple_face_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng
     r = convert_to_primitive(&cb, prm->gamma);
   }
   const double velocity = (nx * wx + ny * wy) / vabs(nx, ny);
-  return rotate_solve_rotate_back(&l, &r, velocity, nx, ny, px, py, out);
+  return rotate_solve_rotate_back(&l, &r, velocity, nx, ny, px, py, out, branch);
 }
 
 ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const double* wx,
@@ -804,13 +904,14 @@ ro_error ro_flux(const ro_mesh* m, const ro_params* prm, ro_cells cng, const dou
     if (nt == 1 && bad_code != RO_OK) continue;
     const double w_x = wx ? wx[f] : 0.0, w_y = wy ? wy[f] : 0.0;
     cons_t c;
-    int rc;
+    int rc, branch = 0;
     if (prm->order == 2 || prm->order == 3) {
       const cc_t lf = get_cell(left, f), rt = get_cell(right, f);
-      rc = modular_face_flux(m, prm, f, &lf, &rt, w_x, w_y, &c);
+      rc = modular_face_flux(m, prm, f, &lf, &rt, w_x, w_y, &c, &branch);
     } else {
-      rc = simple_face_flux(m, prm, cng, f, w_x, w_y, &c);
+      rc = simple_face_flux(m, prm, cng, f, w_x, w_y, &c, &branch);
     }
+    fp_add(2, (uint64_t)f, (uint64_t)branch);
     if (rc != RO_OK) {
 #pragma omp critical
       {
@@ -961,6 +1062,15 @@ real ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells, const double
   const int32_t N = m->n_cells;
   const int nt = prm->nthreads > 1 ? prm->nthreads : 1;
   real res = cfl_cell(m, prm->gamma, cells, wx, wy, 0);
+  if (g_fp) { /* the cell that sets dt (the first of equal minima) */
+    int32_t arg = 0;
+    real best = res;
+    for (int32_t i = 1; i < N; ++i) {
+      const real v = cfl_cell(m, prm->gamma, cells, wx, wy, i);
+      if (v < best) best = v, arg = i;
+    }
+    fp_add(3, (uint64_t)arg, 0);
+  }
   if (nt == 1) {
     for (int32_t i = 1; i < N; ++i) res = smin(res, cfl_cell(m, prm->gamma, cells, wx, wy, i));
   } else {
@@ -1018,6 +1128,9 @@ ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int3
   ro_cells midc = {cb, cb + NG, cb + 2 * (size_t)NG, cb + 3 * (size_t)NG};
   ro_error err = err_make(RO_OK, -1);
   for (int32_t step = 0; step < nsteps && err.code == RO_OK; ++step) {
+    fp_flush();
+    g_fp_step = step;
+    g_fp_stage = 0;
     const real dt = ro_cfl(m, prm, cng, wx, wy);
     if (dt_out) dt_out[step] = RO_VAL(dt);
     if (integrator == 1) {
@@ -1037,6 +1150,7 @@ ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int3
       ro_update_extensives(m, prm, flux, dt, mid);
       err = ro_update_cells(m, prm, mid, midc);
       if (err.code != RO_OK) break;
+      g_fp_stage = 1;
       err = ro_flux(m, prm, midc, wx, wy, flux);
       if (err.code != RO_OK) break;
       zero_force(N, dt, mid);
@@ -1051,11 +1165,24 @@ ro_error ro_run(const ro_mesh* m, const ro_params* prm, int32_t integrator, int3
       err = ro_update_cells(m, prm, ext, cng);
     }
   }
+  fp_flush();
+  g_fp_step = -1;
   free(fb);
   free(mb);
   free(cb);
   return err;
 }
+
+#ifndef RO_TANGENT
+/* Record ro_run's branches into per_step[n][8] (zeroed here); NULL stops. */
+void ro_fingerprint(uint64_t* per_step, int32_t n) {
+  g_fp = per_step;
+  g_fp_n = per_step ? n : 0;
+  g_fp_step = -1;
+  for (int32_t k = 0; k < g_fp_n * RO_FP_KINDS; ++k) per_step[k] = 0;
+  memset(g_fp_part, 0, sizeof(g_fp_part));
+}
+#endif
 
 #ifndef RO_TANGENT
 /* ------------------------------------------------ moving-mesh inputs -- */

@@ -110,6 +110,16 @@ double ro_cfl(const ro_mesh* m, const ro_params* prm, ro_cells cells,
 /* hdsim init_extensives. */
 void ro_init_extensives(const ro_mesh* m, const ro_params* prm, ro_cells cells,
                         ro_ext ext);
+/* Record the branches of each step of ro_run as 64-bit sums of hashes in
+ * per_step[n][8], zeroed here, one sum per kind of branch.  Jumps (the
+ * value is discontinuous there): [6] the shock switch, [7] the limiters'
+ * gates and guards.  Kinks (the derivative is): [2] HLLC's wave pattern and
+ * the sides giving its outer wave speeds, [3] the CFL argmin, [4] and [5]
+ * slope_limit's and shocked_slope_limit's active faces, clamps and
+ * extremum neighbours.  NULL switches the record off.  Equal sums: the
+ * same branches. */
+void ro_fingerprint(uint64_t* per_step, int32_t n);
+
 /* --- moving-mesh inputs (rigid box) --- */
 /* Lagrangian point velocities plus RoundCells::calc_dw without the cold
  * switch: w_i = v_i + chi max(c_i, |v_i|) (CM_i - r_i) / R_i where

@@ -23,8 +23,8 @@ PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_scheme.py",
 # name -> [(text in rich_oracle.c, replacement)], each text found exactly once
 MUTATIONS = {
     # shocked_slope_limit (LinearGaussImproved.cpp:343-366)
-    "shocked guard with abs": [("if (fabs(dphi) > 1e-9 * cell) *psi",
-                                "if (fabs(dphi) > fabs(1e-9 * cell)) *psi")],
+    "shocked guard with abs": [("if (fabs(dphi) > 1e-9 * cell) {",
+                                "if (fabs(dphi) > fabs(1e-9 * cell)) {")],
     "shocked gate 0.05": [("if (fabs(dphi) > 0.1 * smax(", "if (fabs(dphi) > 0.05 * smax(")],
     "theta 1": [("smax(theta * (nval - cell) / dphi, 0.0)", "smax(1.0 * (nval - cell) / dphi, 0.0)")],
     "shocked limiter skips mirror faces": [
