@@ -85,4 +85,71 @@ __global__ void __launch_bounds__(256) k_face_velocity(DMesh m, const D2* __rest
   fwy[f] = (wl.y + wr.y) * 0.5 + (dyr * s) / d2;
 }
 
+// Cell geometry from the face vertices, in each cell's ascending face
+// order: Tessellation::GetVolume, a fan about the generating point
+// (VoronoiMesh.cpp:420-434); SlabSymmetry::calcVolume, a fan of
+// calc_triangle_area anchored at the first face's first vertex
+// (physical_geometry.cpp:13-21, triangle_area.cpp:4-9); and CalcCellCM on
+// the same fan (VoronoiMesh.cpp:436-451).  zcross(v) = (v.y, -v.x).
+__global__ void __launch_bounds__(256) k_cell_geometry(DMesh m, double* __restrict__ tvol,
+                                                       double* __restrict__ vol,
+                                                       D2* __restrict__ cm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.N) return;
+  const int b = m.off[i], e = m.off[i + 1];
+  const double cx = m.px[i], cy = m.py[i];
+  double area = 0;
+  for (int k = b; k < e; ++k) {
+    const int f = static_cast<int>(m.inc[k].info & INFO_FACE);
+    const double p1x = m.v1x[f] - cx, p1y = m.v1y[f] - cy;
+    const double p2x = m.v2x[f] - cx, p2y = m.v2y[f] - cy;
+    area += 0.5 * fabs(p1x * p2y + p1y * (-p2x));
+  }
+  double v = 0, a2 = 0, pcx = 0, pcy = 0;
+  if (e > b) {
+    const int f0 = static_cast<int>(m.inc[b].info & INFO_FACE);
+    const double ax = m.v1x[f0], ay = m.v1y[f0];
+    for (int k = b + 1; k < e; ++k) {
+      const int f = static_cast<int>(m.inc[k].info & INFO_FACE);
+      const double q1x = m.v1x[f] - ax, q1y = m.v1y[f] - ay;
+      const double q2x = m.v2x[f] - ax, q2y = m.v2y[f] - ay;
+      const double at = 0.5 * fabs(q1x * q2y + q1y * (-q2x));
+      v += at;
+      a2 += at;
+      const double s = at / 3.;
+      pcx += ((ax + m.v1x[f]) + m.v2x[f]) * s;
+      pcy += ((ay + m.v1y[f]) + m.v2y[f]) * s;
+    }
+  }
+  tvol[i] = area;
+  vol[i] = v;
+  cm[i] = D2{pcx / a2, pcy / a2};
+}
+
+// Centres of mass of the ghosts (VoronoiMesh.cpp:359-398): a periodic copy
+// is its original's CM shifted with its point; a rigid-wall ghost is its
+// original's CM reflected across the wall its point lies beyond
+// (GetReflection, VoronoiMesh.cpp:316-330: point + (2 (wall - x), 0) or
+// (0, 2 (wall - y))).
+__global__ void __launch_bounds__(256) k_ghost_cm(DMesh m, D2* __restrict__ cm, double x0,
+                                                  double x1, double y0, double y1) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m.G) return;
+  const int id = m.N + g, o = m.gorig[g];
+  const D2 c = cm[o];
+  if (m.bc == BC_PERIODIC) {
+    cm[id] = D2{c.x + (m.px[id] - m.px[o]), c.y + (m.py[id] - m.py[o])};
+    return;
+  }
+  const double gx = m.px[id], gy = m.py[id];
+  if (gx > x1)
+    cm[id] = D2{c.x + 2 * (x1 - c.x), c.y + 0.0};
+  else if (gy > y1)
+    cm[id] = D2{c.x + 0.0, c.y + 2 * (y1 - c.y)};
+  else if (gx < x0)
+    cm[id] = D2{c.x + 2 * (x0 - c.x), c.y + 0.0};
+  else
+    cm[id] = D2{c.x + 0.0, c.y + 2 * (y0 - c.y)};
+}
+
 }  // namespace rg

@@ -193,6 +193,10 @@ struct rg_ctx {
   size_t keep_cap = 0;
   bool pv_valid = false;
 
+  // the rigid box (x0, x1, y0, y1) of rg_set_box, for device geometry
+  double box[4] = {0, 0, 0, 0};
+  bool has_box = false;
+
   // device of the context's stream and its copy of the noalloc scratch
   int device = 0;
   double* scratch = nullptr;
@@ -973,9 +977,14 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   const int N = m->n_cells, F = m->n_faces, G = m->n_ghosts;
   if (N <= 0 || F <= 0 || G < 0 || F > static_cast<int>(INFO_FACE)) return RG_E_ARG;
   if (!m->face_first || !m->face_second || !m->v1x || !m->v1y || !m->v2x || !m->v2y ||
-      !m->pt_x || !m->pt_y || !m->cm_x || !m->cm_y || !m->volume || !m->tess_volume ||
-      !m->csr_off || !m->csr_face || (G > 0 && !m->ghost_orig))
+      !m->pt_x || !m->pt_y || !m->csr_off || !m->csr_face || (G > 0 && !m->ghost_orig))
     return RG_E_ARG;
+  // the cell geometry comes with the mesh, or (all four NULL) is computed on
+  // the device; a rigid box's ghosts then need the box
+  const int n_geo = !!m->cm_x + !!m->cm_y + !!m->volume + !!m->tess_volume;
+  if (n_geo != 0 && n_geo != 4) return RG_E_ARG;
+  const bool dev_geo = n_geo == 0;
+  if (dev_geo && G > 0 && c->prm.bc == RG_BC_RIGID && !c->has_box) return RG_E_ARG;
   // --- validate topology on the host threads
   if (m->csr_off[0] != 0) return RG_E_ARG;
   for (int i = 0; i < N; ++i)
@@ -1078,9 +1087,10 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       gn[g] = D2{nx / an, ny / an};
     }
   }
-  std::vector<D2> cm(static_cast<size_t>(N) + G);
-  rmp::parallel_for(static_cast<int64_t>(N) + G,
-                    [&](int64_t i) { cm[i] = D2{m->cm_x[i], m->cm_y[i]}; });
+  std::vector<D2> cm(dev_geo ? 0 : static_cast<size_t>(N) + G);
+  if (!dev_geo)
+    rmp::parallel_for(static_cast<int64_t>(N) + G,
+                      [&](int64_t i) { cm[i] = D2{m->cm_x[i], m->cm_y[i]}; });
   // --- upload
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drain_pool(c);
@@ -1107,11 +1117,17 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, upload(c, &v2y, m->v2y, F));
   HIPCHK(c, upload(c, &px, m->pt_x, NG));
   HIPCHK(c, upload(c, &py, m->pt_y, NG));
-  HIPCHK(c, upload(c, &tvol, m->tess_volume, N));
-  HIPCHK(c, upload(c, &vol, m->volume, N));
+  if (dev_geo) {
+    HIPCHK(c, dalloc(c, &tvol, N));
+    HIPCHK(c, dalloc(c, &vol, N));
+    HIPCHK(c, dalloc(c, &cm_d, NG));
+  } else {
+    HIPCHK(c, upload(c, &tvol, m->tess_volume, N));
+    HIPCHK(c, upload(c, &vol, m->volume, N));
+    HIPCHK(c, upload(c, &cm_d, cm.data(), NG));
+  }
   HIPCHK(c, upload(c, &off, m->csr_off, static_cast<size_t>(N) + 1));
   HIPCHK(c, upload(c, &inc_d, inc.data(), static_cast<size_t>(nnz)));
-  HIPCHK(c, upload(c, &cm_d, cm.data(), NG));
   HIPCHK(c, upload(c, &gorig, G ? m->ghost_orig : &zero, G ? G : 1));
   HIPCHK(c, upload(c, &gsrc_d, gsrc.data(), gsrc.size()));
   HIPCHK(c, upload(c, &gn_d, gn.data(), gn.size()));
@@ -1181,6 +1197,17 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   d.eh = eh;
   d.eo = eo;
   d.earea = earea;
+  if (dev_geo) {
+    // volumes and centres of mass from the vertices, then the ghosts' CMs
+    HIPCHK(c, launch(c, KID_GEO, [&] {
+      k_cell_geometry<<<blocks_for(N), kBlock, 0, c->stream>>>(d, tvol, vol, cm_d);
+    }));
+    if (G > 0)
+      HIPCHK(c, launch(c, KID_GEO, [&] {
+        k_ghost_cm<<<blocks_for(G), kBlock, 0, c->stream>>>(d, cm_d, c->box[0], c->box[1],
+                                                           c->box[2], c->box[3]);
+      }));
+  }
   // static geometry tables
   HIPCHK(c, launch(c, KID_GEO, [&] {
     k_geo_faces<<<blocks_for(F), kBlock, 0, c->stream>>>(d, frec, fframe, area);
@@ -1883,6 +1910,37 @@ int rg_ensemble_autotune(rg_ctx* c, int32_t trial_steps, double* table, int32_t 
   HIPCHK(c, ens_capture(c, best_block, best_bpc));
   if (rows_out) *rows_out = rows;
   return collect_error(c);
+}
+
+int rg_set_box(rg_ctx* c, const double* box) {
+  if (!c || !box || !(box[0] < box[1]) || !(box[2] < box[3])) return RG_E_ARG;
+  for (int k = 0; k < 4; ++k) c->box[k] = box[k];
+  c->has_box = true;
+  return RG_OK;
+}
+
+int rg_get_geometry(rg_ctx* c, double* volume, double* tess_volume, double* cm_x,
+                    double* cm_y) {
+  if (!c || !c->has_mesh) return RG_E_ARG;
+  const size_t N = static_cast<size_t>(c->N), NG = N + static_cast<size_t>(c->G);
+  if (volume)
+    HIPCHK(c, hipMemcpyAsync(volume, c->dm.vol, sizeof(double) * N, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (tess_volume)
+    HIPCHK(c, hipMemcpyAsync(tess_volume, c->dm.tvol, sizeof(double) * N, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (cm_x || cm_y) {
+    std::vector<D2> cm(NG);
+    HIPCHK(c, hipMemcpyAsync(cm.data(), c->dm.cm, sizeof(D2) * NG, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < NG; ++i) {
+      if (cm_x) cm_x[i] = cm[i].x;
+      if (cm_y) cm_y[i] = cm[i].y;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RG_OK;
 }
 
 int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {

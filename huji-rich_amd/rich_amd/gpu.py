@@ -69,6 +69,8 @@ def lib():
     L.rg_set_stream.argtypes = [vp, vp]
     L.rg_set_mesh.argtypes = [vp, P(MeshS)]
     L.rg_set_face_velocity.argtypes = [vp, vp, vp]
+    L.rg_set_box.argtypes = [vp, vp]
+    L.rg_get_geometry.argtypes = [vp, vp, vp, vp, vp]
     L.rg_replace_mesh.argtypes = [vp, P(MeshS)]
     L.rg_moving_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp,
                                   P(ctypes.c_double), vp, vp]
@@ -113,8 +115,12 @@ class RichGPU:
     extensives and fluxes (4, n) = mass, energy, momentum x, momentum y;
     slopes (8, N) = d/dx of the four cell fields, then d/dy."""
 
+    GEOMETRY = ("cm_x", "cm_y", "volume", "tess_volume")
+
     def __init__(self, mesh, gamma=5.0 / 3.0, cfl=0.3, order=2, slf=True, delta_v=0.2,
-                 theta=0.5, delta_p=0.7, precision=RG_FP64):
+                 theta=0.5, delta_p=0.7, precision=RG_FP64, device_geometry=False, box=None):
+        """device_geometry: upload no volumes or centres of mass and let the
+        device form them (a rigid mesh then needs its box (x0, x1, y0, y1))."""
         self._lib = lib()
         self.mesh = mesh
         self.N, self.F = mesh.n_cells, mesh.n_faces
@@ -123,12 +129,31 @@ class RichGPU:
         h = ctypes.c_void_p()
         self._check(self._lib.rg_create(ctypes.byref(prm), ctypes.byref(h)), handle=None)
         self._h = h
+        self.device_geometry = device_geometry
+        if box is not None:
+            self.set_box(box)
         self._upload(self._lib.rg_set_mesh, mesh)
+
+    def set_box(self, box):
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        if b.shape != (4,):
+            raise ValueError("box is (x0, x1, y0, y1)")
+        self._check(self._lib.rg_set_box(self._h, _p(b)))
+
+    def geometry(self):
+        """(volume, tess_volume, cm_x, cm_y) as the device holds them."""
+        N, NG = self.N, self.N + self.mesh.n_ghosts
+        v, tv, cx, cy = np.empty(N), np.empty(N), np.empty(NG), np.empty(NG)
+        self._check(self._lib.rg_get_geometry(self._h, _p(v), _p(tv), _p(cx), _p(cy)))
+        return v, tv, cx, cy
 
     def _upload(self, fn, mesh):
         keep = []
         ptrs = []
         for k in MESH_FIELDS:
+            if self.device_geometry and k in self.GEOMETRY:
+                ptrs.append(None)
+                continue
             a = np.ascontiguousarray(getattr(mesh, k))
             if a.size == 0:
                 a = np.zeros(1, dtype=a.dtype)

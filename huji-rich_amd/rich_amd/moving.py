@@ -212,13 +212,17 @@ class DeviceMovingMesh:
     """A moving-mesh run with the state on the GPU; `step()` advances one
     TimeAdvance2Heun step, `cells()` reads the cells back."""
 
-    def __init__(self, x, y, cells, box, gamma=5.0 / 3.0, cfl=0.3, chi=0.15, eta=0.02):
+    def __init__(self, x, y, cells, box, gamma=5.0 / 3.0, cfl=0.3, chi=0.15, eta=0.02,
+                 device_geometry=True):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.chi, self.eta = tuple(float(v) for v in box), chi, eta
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
-        self.g = RichGPU(self.mesh, gamma=gamma, cfl=cfl, order=2)
+        # volumes and centres of mass formed on the device from each mesh's
+        # vertices (not uploaded)
+        self.g = RichGPU(self.mesh, gamma=gamma, cfl=cfl, order=2,
+                         device_geometry=device_geometry, box=self.box)
         self.g.set_cells(np.asarray(cells, dtype=np.float64))
         self.g.init_extensives()
         self.time = 0.0

@@ -89,6 +89,13 @@ typedef struct rg_mesh {
   const double *cm_x, *cm_y;               /* [N+G] GetCellCM                */
   const double *volume;                    /* [N] CacheData::volumes         */
   const double *tess_volume;               /* [N] GetVolume (width, slopes)  */
+  /* cm_x, cm_y, volume, tess_volume may all be NULL: the device then forms
+   * them from the vertices in each cell's face order (CalcCellCM
+   * VoronoiMesh.cpp:436-451, SlabSymmetry::calcVolume
+   * physical_geometry.cpp:13-21, GetVolume VoronoiMesh.cpp:420-434) and
+   * the ghosts' CMs from their originals' (shifted for a periodic box,
+   * reflected across the wall each ghost lies beyond for a rigid one, which
+   * needs rg_set_box; VoronoiMesh.cpp:359-398). */
   const int32_t *csr_off;                  /* [N+1]                          */
   const int32_t *csr_face;                 /* [csr_off[N]] GetCellEdges      */
   const int32_t *ghost_orig;               /* [G] GetOriginalIndex           */
@@ -137,6 +144,13 @@ int rg_set_stream(rg_ctx* ctx, void* hip_stream);
  * cell lists faces it is a side of, in ascending order; every face between
  * two real cells is listed by both; ghost_orig is in [0, N). */
 int rg_set_mesh(rg_ctx* ctx, const rg_mesh* mesh);
+/* The rigid box [x0, x1, y0, y1] (SquareBox, the OuterBoundary of a rigid
+ * run): needed only to form a rigid mesh's geometry on the device. */
+int rg_set_box(rg_ctx* ctx, const double* box);
+/* The mesh geometry the device holds: volume [N], tess_volume [N],
+ * cm_x/cm_y [N+G] (any may be NULL). */
+int rg_get_geometry(rg_ctx* ctx, double* volume, double* tess_volume, double* cm_x,
+                    double* cm_y);
 /* Face velocities [F] as hdsim passes edge_velocities; NULL,NULL = all zero
  * (StationaryBox/PeriodicEdgeVelocities on a static mesh). */
 int rg_set_face_velocity(rg_ctx* ctx, const double* wx, const double* wy);
