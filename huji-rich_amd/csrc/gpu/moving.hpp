@@ -152,4 +152,13 @@ __global__ void __launch_bounds__(256) k_ghost_cm(DMesh m, D2* __restrict__ cm, 
     cm[id] = D2{c.x + 0.0, c.y + 2 * (y0 - c.y)};
 }
 
+// dst[i] = src[order[i]] (VectorValues, misc/utils.hpp): the state follows
+// the cells into a new order
+template <class T>
+__global__ void __launch_bounds__(256) k_gather(T* __restrict__ dst, const T* __restrict__ src,
+                                               const int* __restrict__ order, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[order[i]];
+}
+
 }  // namespace rg

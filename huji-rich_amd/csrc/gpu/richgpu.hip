@@ -1943,6 +1943,50 @@ int rg_get_geometry(rg_ctx* c, double* volume, double* tess_volume, double* cm_x
   return RG_OK;
 }
 
+int rg_permute_cells(rg_ctx* c, const int32_t* order) {
+  if (!ready(c, true, true) || !order) return RG_E_ARG;
+  const int N = c->N;
+  {
+    std::vector<char> seen(static_cast<size_t>(N), 0);
+    for (int i = 0; i < N; ++i) {
+      const int o = order[i];
+      if (o < 0 || o >= N || seen[o]) return RG_E_ARG;
+      seen[o] = 1;
+    }
+  }
+  const size_t bytes = sizeof(D4) * static_cast<size_t>(N);
+  if (c->keep_cap < 3 * static_cast<size_t>(N)) {
+    if (c->keep) (void)hipFree(c->keep);
+    c->keep = nullptr;
+    c->keep_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->keep), 3 * bytes));
+    c->keep_cap = 3 * static_cast<size_t>(N);
+  }
+  // the order goes through the staging buffer (4 max(N, F) doubles)
+  int* ord = reinterpret_cast<int*>(c->staging);
+  HIPCHK(c, hipMemcpyAsync(ord, order, sizeof(int) * N, hipMemcpyHostToDevice, c->stream));
+  D4* const arrays[3] = {c->prim, c->ext, c->mid_ext};
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(c, launch(c, KID_MOVING, [&] {
+      k_gather<D4><<<blocks_for(N), kBlock, 0, c->stream>>>(c->keep + static_cast<size_t>(k) * N,
+                                                            arrays[k], ord, N);
+    }));
+  }
+  for (int k = 0; k < 3; ++k)
+    HIPCHK(c, hipMemcpyAsync(arrays[k], c->keep + static_cast<size_t>(k) * N, bytes,
+                             hipMemcpyDeviceToDevice, c->stream));
+  if (c->pv_valid && c->pv) {
+    D2* tmp = reinterpret_cast<D2*>(c->keep);
+    HIPCHK(c, launch(c, KID_MOVING, [&] {
+      k_gather<D2><<<blocks_for(N), kBlock, 0, c->stream>>>(tmp, c->pv, ord, N);
+    }));
+    HIPCHK(c, hipMemcpyAsync(c->pv, tmp, sizeof(D2) * N, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ext_init = false;
+  return RG_OK;
+}
+
 int rg_replace_mesh(rg_ctx* c, const rg_mesh* m) {
   if (!c || !m || !c->has_mesh || !c->has_cells || !c->has_ext || m->n_cells != c->N)
     return RG_E_ARG;

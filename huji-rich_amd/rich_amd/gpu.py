@@ -72,6 +72,7 @@ def lib():
     L.rg_set_box.argtypes = [vp, vp]
     L.rg_get_geometry.argtypes = [vp, vp, vp, vp, vp]
     L.rg_replace_mesh.argtypes = [vp, P(MeshS)]
+    L.rg_permute_cells.argtypes = [vp, vp]
     L.rg_moving_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp,
                                   P(ctypes.c_double), vp, vp]
     L.rg_moving_end.argtypes = [vp]
@@ -217,6 +218,15 @@ class RichGPU:
         self._upload(self._lib.rg_replace_mesh, mesh)
         self.mesh = mesh
         self.F = mesh.n_faces
+
+    def permute_cells(self, order):
+        """New cell i = old cell order[i]: cells, extensives, mid extensives
+        and point velocities follow (then replace_mesh with the mesh of the
+        reordered points)."""
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        if o.shape != (self.N,):
+            raise ValueError(f"expected {self.N} indices")
+        self._check(self._lib.rg_permute_cells(self._h, _p(o)))
 
     def moving_end(self):
         """Stage 2 of the moving-mesh step on the replaced mesh."""

@@ -17,8 +17,12 @@ points and rebuilds the mesh (rg_replace_mesh).  Both give the same values.
 Both rebuild the mesh with a `BoxMesher`, which repairs the previous
 step's Delaunay triangulation by edge flips where it can.
 
-Cells keep the order of their points: the Hilbert re-sort that hdsim
-applies every 25 cycles only permutes cells and is not done."""
+Every `resort_every` cycles (25, as hdsim: MoveMeshPoints with reorder on
+cycle % 25 == 0, hdsim2d.cpp:389-399) the moved points are put in Hilbert
+order and the cells, extensives, mid extensives and point velocities follow
+them (`hilbert_order`: the textbook curve on a 2^16 grid over the box, not
+the reference's adaptive HilbertOrder, so the order can differ from
+hdsim's)."""
 import time
 
 import numpy as np
@@ -114,6 +118,41 @@ def correct_overshoot(mesh, vx, vy, dt, box):
     return vx, vy
 
 
+def hilbert_order(x, y, box, bits=16):
+    """Indices that sort the points along a Hilbert curve through the box
+    (position on the curve of each point's cell in a 2^bits x 2^bits grid;
+    ties in the order given)."""
+    x0, x1, y0, y1 = box
+    n = 1 << bits
+    xi = np.clip(((np.asarray(x) - x0) / (x1 - x0) * n).astype(np.int64), 0, n - 1)
+    yi = np.clip(((np.asarray(y) - y0) / (y1 - y0) * n).astype(np.int64), 0, n - 1)
+    d = np.zeros(len(xi), dtype=np.int64)
+    s = n // 2
+    while s > 0:
+        rx = (xi & s) > 0
+        ry = (yi & s) > 0
+        d += s * s * ((3 * rx.astype(np.int64)) ^ ry.astype(np.int64))
+        # rotate the quadrant
+        flip = ~ry & rx
+        xi = np.where(flip, n - 1 - xi, xi)
+        yi = np.where(flip, n - 1 - yi, yi)
+        xi, yi = np.where(~ry, yi, xi), np.where(~ry, xi, yi)
+        s //= 2
+    return np.argsort(d, kind="stable")
+
+
+def _resort_order(self):
+    """On a re-sort cycle, the Hilbert order of the moved points if it
+    differs from the current one (counted in `reorders`), else None."""
+    if not self.resort_every or self.cycle % self.resort_every:
+        return None
+    order = hilbert_order(self.x, self.y, self.box)
+    if np.array_equal(order, np.arange(len(order))):
+        return None
+    self.reorders += 1
+    return order
+
+
 class HostMotion:
     """The point motion of the run on the host: Lagrangian + RoundCells
     point velocities, the overshoot fix and StationaryBox face velocities."""
@@ -137,11 +176,13 @@ class MovingMesh:
     between steps; `step()` advances one TimeAdvance2Heun step.  `motion`
     supplies the point and face velocities (default HostMotion)."""
 
-    def __init__(self, x, y, cells, box, backend, gamma=5.0 / 3.0, motion=None):
+    def __init__(self, x, y, cells, box, backend, gamma=5.0 / 3.0, motion=None,
+                 resort_every=25):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.b, self.gamma = box, backend, gamma
         self.motion = motion if motion is not None else HostMotion(gamma)
+        self.resort_every, self.cycle, self.reorders = resort_every, 0, 0
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
         self.b.set_mesh(self.mesh)
@@ -149,6 +190,8 @@ class MovingMesh:
         self.ext = self.b.init_extensives(self.cells)
         self.time = 0.0
         self.timing = {"mesh_s": 0.0, "motion_s": 0.0, "plugins_s": 0.0}
+
+    _resort_order = _resort_order
 
     def _plug(self, fn, *a):
         t = time.perf_counter()
@@ -178,10 +221,16 @@ class MovingMesh:
         mid[3] += 0.0 * dt
         mid[1] += dt * 0.0
         mid = self._plug(b.update_extensives, flux, dt, mid)
-        # MoveMeshPoints (hydrodynamics_2d.cpp:218-247) and the rebuild
+        # MoveMeshPoints (hydrodynamics_2d.cpp:218-247), the Hilbert re-sort
+        # every resort_every cycles, and the rebuild
         t = time.perf_counter()
         self.x = self.x + dt * vx
         self.y = self.y + dt * vy
+        order = self._resort_order()
+        if order is not None:
+            self.x, self.y, vx, vy = self.x[order], self.y[order], vx[order], vy[order]
+            self.cells, self.ext, mid = self.cells[:, order], self.ext[:, order], mid[:, order]
+            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
         self.mesh = self.mesher.mesh(self.x, self.y)
         T["mesh_s"] += time.perf_counter() - t
         self._plug(b.set_mesh, self.mesh)
@@ -205,6 +254,7 @@ class MovingMesh:
         self.ext = ext
         self.cells = self._plug(b.update_cells, ext)
         self.time += dt
+        self.cycle += 1
         return dt
 
 
@@ -212,11 +262,14 @@ class DeviceMovingMesh:
     """A moving-mesh run with the state on the GPU; `step()` advances one
     TimeAdvance2Heun step, `cells()` reads the cells back."""
 
+    _resort_order = _resort_order
+
     def __init__(self, x, y, cells, box, gamma=5.0 / 3.0, cfl=0.3, chi=0.15, eta=0.02,
-                 device_geometry=True):
+                 device_geometry=True, resort_every=25):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.chi, self.eta = tuple(float(v) for v in box), chi, eta
+        self.resort_every, self.cycle, self.reorders = resort_every, 0, 0
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
         # volumes and centres of mass formed on the device from each mesh's
@@ -233,9 +286,16 @@ class DeviceMovingMesh:
         t0 = time.perf_counter()
         dt, vx, vy = self.g.moving_begin(self.chi, self.eta, self.box)
         t1 = time.perf_counter()
-        # MoveMeshPoints (hydrodynamics_2d.cpp:218-247) and the rebuild
+        # MoveMeshPoints (hydrodynamics_2d.cpp:218-247), the Hilbert re-sort
+        # every resort_every cycles (the state follows on the device), and
+        # the rebuild
         self.x = self.x + dt * vx
         self.y = self.y + dt * vy
+        order = self._resort_order()
+        if order is not None:
+            self.x, self.y = self.x[order], self.y[order]
+            self.g.permute_cells(order)
+            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
         self.mesh = self.mesher.mesh(self.x, self.y)
         t2 = time.perf_counter()
         self.g.replace_mesh(self.mesh)
@@ -247,6 +307,7 @@ class DeviceMovingMesh:
         T["replace_s"] += t3 - t2
         T["end_s"] += t4 - t3
         self.time += dt
+        self.cycle += 1
         return dt
 
     def cells(self):

@@ -171,6 +171,13 @@ int rg_moving_begin(rg_ctx* ctx, double chi, double eta, const double* box, doub
  * hdsim2d.cpp:390; same cell count and order): replaces the mesh and keeps
  * cells, extensives, mid extensives and point velocities. */
 int rg_replace_mesh(rg_ctx* ctx, const rg_mesh* mesh);
+/* The cells in a new order, new cell i = old cell order[i] (a permutation
+ * of [0, N)): cells, extensives, mid extensives and point velocities follow,
+ * as hdsim permutes cells_, extensives_, mid_extensives and
+ * point_velocities with the Hilbert order MoveMeshPoints returns every 25
+ * cycles (hdsim2d.cpp:389-399).  The caller then gives the mesh of the
+ * reordered points to rg_replace_mesh. */
+int rg_permute_cells(rg_ctx* ctx, const int32_t* order);
 /* Stage 2 on the replaced mesh: mid cells, edge velocities from the kept
  * point velocities, fluxes, the update, average_extensive and the cell
  * update (hdsim2d.cpp:401-441). */

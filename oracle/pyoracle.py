@@ -143,9 +143,12 @@ class Oracle:
         self._keep = [np.ascontiguousarray(getattr(mesh, k)) for k in (
             "face_first", "face_second", "v1x", "v1y", "v2x", "v2y", "pt_x", "pt_y", "cm_x",
             "cm_y", "volume", "tess_volume", "csr_off", "csr_face", "ghost_orig")]
+        # an empty ghost map still gets a valid pointer; every array handed
+        # to the oracle stays referenced by this object while it is in use
         gorig = self._keep[-1]
+        self._one = np.zeros(1, np.int32)
         self.m = Mesh_(mesh.n_cells, mesh.n_faces, mesh.n_ghosts,
-                       *[_p(a) if a.size else (_p(np.zeros(1, np.int32)) if a is gorig else None)
+                       *[_p(a) if a.size else (_p(self._one) if a is gorig else None)
                          for a in self._keep])
         self.prm = Params_(gamma, cfl, order, mesh.bc, 1 if slf else 0, delta_v, theta,
                            delta_p, nthreads)
@@ -225,14 +228,16 @@ class Oracle:
 
     def update_extensives(self, flux, dt, ext):
         ext = np.array(ext, dtype=np.float64, order="C")
+        fl = np.ascontiguousarray(flux, dtype=np.float64)  # held through the call
         lib().ro_update_extensives(ctypes.byref(self.m), ctypes.byref(self.prm),
-                                   _quad(np.ascontiguousarray(flux)), dt, _quad(ext))
+                                   _quad(fl), dt, _quad(ext))
         return ext
 
     def update_cells(self, ext):
         cells = np.zeros((4, self.N))
+        x = np.ascontiguousarray(ext, dtype=np.float64)  # held through the call
         e = lib().ro_update_cells(ctypes.byref(self.m), ctypes.byref(self.prm),
-                                  _quad(np.ascontiguousarray(ext)), _quad(cells))
+                                  _quad(x), _quad(cells))
         self._check(e)
         return cells
 
@@ -244,8 +249,9 @@ class Oracle:
 
     def init_extensives(self, cells):
         ext = np.zeros((4, self.N))
+        c = np.ascontiguousarray(cells[:, :self.N], dtype=np.float64)  # held through the call
         lib().ro_init_extensives(ctypes.byref(self.m), ctypes.byref(self.prm),
-                                 _quad(np.ascontiguousarray(cells[:, :self.N])), _quad(ext))
+                                 _quad(c), _quad(ext))
         return ext
 
     def run(self, cells, ext, nsteps, integrator=2, w=None, fingerprint=False):

@@ -159,3 +159,17 @@ def tiny_velocity_cells(m, shocked, seed=0):
     c[2] = -1.0 + 1e-11 * rng.random(N)
     c[3] = 1.0 + 1e-11 * rng.random(N)
     return c
+
+
+def smooth_box_case(n=24, seed=5):
+    """A jittered lattice in a rigid box with smooth, subsonic states: no
+    shock and no limiter switch, so reordering sums changes the run only by
+    rounding (a shock amplifies such differences through the limiter's
+    switches)."""
+    m = rmesh.lattice(n, n, (-1.0, 1.0, -1.0, 1.0), jitter=5e-2, seed=seed, bc=rmesh.RIGID)
+    N = m.n_cells
+    x, y = m.cm_x[:N], m.cm_y[:N]
+    c = np.vstack([1 + 0.2 * np.exp(-8 * (x * x + y * y)),
+                   1 + 0.3 * np.exp(-8 * ((x - 0.2) ** 2 + y * y)),
+                   0.1 * np.sin(2 * y), 0.1 * np.cos(3 * x)])
+    return m, c

@@ -49,7 +49,7 @@ class OracleMotion:
         return fw[0], fw[1]
 
 
-def oracle_moving_mesh(x, y, cells, box):
+def oracle_moving_mesh(x, y, cells, box, **kw):
     """A MovingMesh whose plugins and point motion are the oracle's."""
     from rich_amd.moving import MovingMesh
-    return MovingMesh(x, y, cells, box, OracleBackend(), motion=OracleMotion())
+    return MovingMesh(x, y, cells, box, OracleBackend(), motion=OracleMotion(), **kw)
