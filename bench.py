@@ -150,6 +150,7 @@ def moving_leg(args):
            "replace_mesh_s_per_step": T["replace_s"] / steps,
            "end_s_per_step": T["end_s"] / steps,
            "mesh_builds_repairs": list(dev.mesher.counts()),
+           "hilbert_resort": {"every": dev.resort_every, "reorders": dev.reorders},
            "mass_drift": float(x1[0].sum() / x0[0].sum() - 1.0),
            "energy_drift": float(x1[1].sum() / x0[1].sum() - 1.0),
            "setup_s": setup,
@@ -383,8 +384,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=2,
-                    help="Heun steps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=40,
+                    help="Heun steps of the CPU baseline sample, about 10 s on 16 host "
+                         "threads (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the per-kernel HIP-event run after the timed one")
