@@ -173,3 +173,41 @@ def smooth_box_case(n=24, seed=5):
                    1 + 0.3 * np.exp(-8 * ((x - 0.2) ** 2 + y * y)),
                    0.1 * np.sin(2 * y), 0.1 * np.cos(3 * x)])
     return m, c
+
+
+def wheel_mesh(k, n=12, radius=0.08):
+    """A rigid unit box whose centre cell has `k` faces: a point at the
+    centre, a ring of k points around it, and an n x n lattice outside a
+    hole around the ring."""
+    g = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(g, g)
+    X, Y = X.ravel(), Y.ravel()
+    keep = np.hypot(X - 0.5, Y - 0.5) > 0.2
+    th = 2 * np.pi * np.arange(k) / k
+    x = np.concatenate([[0.5], 0.5 + radius * np.cos(th), X[keep]])
+    y = np.concatenate([[0.5], 0.5 + radius * np.sin(th), Y[keep]])
+    return rmesh.points(x, y, (0.0, 1.0, 0.0, 1.0))
+
+
+def singular_cell_mesh(i=5, seed=9):
+    """A periodic jittered lattice whose cell i has its GetVolume set to a
+    real eigenvalue of the geometric sum of calc_naive_slope
+    (LinearGaussImproved.cpp:79-112), so that its matrix vol*I - sum c r^T
+    is singular to rounding.  Returns (mesh, i, eigenvalue)."""
+    m = rmesh.lattice(16, 16, (0.0, 1.0, 0.0, 1.0), jitter=5e-2, seed=seed, bc=rmesh.PERIODIC)
+    A = np.zeros((2, 2))
+    for f in m.csr_face[m.csr_off[i]:m.csr_off[i + 1]]:
+        a, b = m.face_first[f], m.face_second[f]
+        j = b if a == i else a
+        cx = (m.v1x[f] + m.v2x[f]) * 0.5 - (m.cm_x[j] + m.cm_x[i]) * 0.5
+        cy = (m.v1y[f] + m.v2y[f]) * 0.5 - (m.cm_y[j] + m.cm_y[i]) * 0.5
+        L = np.hypot(m.v2x[f] - m.v1x[f], m.v2y[f] - m.v1y[f])
+        dx, dy = m.pt_x[j] - m.pt_x[i], m.pt_y[j] - m.pt_y[i]
+        r = np.array([dx, dy]) / np.hypot(dx, dy) * L
+        A += np.outer(r, [cx, cy])
+    ev = np.linalg.eigvals(A)
+    assert np.all(np.isreal(ev)) and ev.real.max() > 0
+    lam = float(ev.real.max())
+    tv = m.tess_volume.copy()
+    tv[i] = lam
+    return replace(m, tess_volume=tv, _handle=None), i, lam

@@ -143,3 +143,57 @@ def test_failed_step_keeps_the_state_and_reports_entries():
     np.testing.assert_array_equal(g.get_cells(), cells)
     np.testing.assert_array_equal(g.get_extensives(), ext)
     g.close()
+
+
+@pytest.mark.parametrize("k", [20, 21])
+def test_neighbour_limit(k):
+    """calc_naive_slope refuses a cell with more than 20 faces
+    (LinearGaussImproved.cpp:72-79): "Cell has too many neighbors" with the
+    cell's point; 20 faces are accepted, with the oracle's slopes."""
+    from rich_amd.gpu import RichGPU, RichGPUError
+    m = cases.wheel_mesh(k)
+    N = m.n_cells
+    cells = cases.rough_cells(m, seed=4)
+    cells[0] = np.abs(cells[0]) + 0.5
+    cells[1] = np.abs(cells[1]) + 0.5
+    g = RichGPU(m, order=2)
+    g.set_cells(cells)
+    if k <= 20:
+        s, _, L, R = po.Oracle(m, order=2).lgi(cells)
+        gs, gl, gr = g.reconstruct()
+        np.testing.assert_array_equal(gs, s)
+        np.testing.assert_array_equal(gl, L)
+        g.close()
+        return
+    for call in (g.reconstruct, g.flux):
+        with pytest.raises(RichGPUError) as e:
+            call()
+        assert e.value.text == "Cell has too many neighbors" and e.value.index == 0
+        assert e.value.entries == [("Cell x cor", m.pt_x[0]), ("Cell y cor", m.pt_y[0])]
+    g.close()
+    assert N > k
+
+
+def test_singular_matrix_entries():
+    """A gradient matrix singular to rounding (LinearGaussImproved.cpp:105-118):
+    "Singular matrix" with the cell's point, its volume and the determinant,
+    as the oracle fails on the same cell."""
+    from rich_amd.gpu import RichGPU, RichGPUError
+    m, i, lam = cases.singular_cell_mesh()
+    cells = cases.rough_cells(m, seed=2)
+    cells[0] = np.abs(cells[0]) + 0.5
+    cells[1] = np.abs(cells[1]) + 0.5
+    with pytest.raises(po.OracleError) as eo:
+        po.Oracle(m, order=2).lgi(cells)
+    g = RichGPU(m, order=2)
+    g.set_cells(cells)
+    with pytest.raises(RichGPUError) as e:
+        g.reconstruct()
+    assert e.value.text == "Singular matrix" == po.CODES[eo.value.code]
+    assert e.value.index == i == eo.value.index
+    names = [n for n, _ in e.value.entries]
+    vals = [v for _, v in e.value.entries]
+    assert names == ["Cell x cor", "Cell y cor", "Cell volume", "Det was"]
+    assert vals[:3] == [m.pt_x[i], m.pt_y[i], lam]
+    assert abs(vals[3]) < 1e-10 * lam * lam
+    g.close()

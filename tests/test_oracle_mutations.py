@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "rich_oracle.c")
 PIN_TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle_scheme.py",
              "tests/test_oracle_hllc.py", "tests/test_oracle_pcm.py",
-             "tests/test_oracle_moving.py"]
+             "tests/test_oracle_moving.py", "tests/test_oracle_limits.py"]
 
 # name -> [(text in rich_oracle.c, replacement)], each text found exactly once
 MUTATIONS = {
@@ -40,6 +40,9 @@ MUTATIONS = {
     "no mirror skip": [
         ("    if (orig_index(m, m->face_first[f]) == orig_index(m, m->face_second[f])) continue;\n",
          "")],
+    # calc_naive_slope's neighbour limit (LinearGaussImproved.cpp:73-78)
+    "neighbour limit at 20": [("if (n > 20) return RO_TOO_MANY_NEIGHBORS;",
+                               "if (n >= 20) return RO_TOO_MANY_NEIGHBORS;")],
     # is_shock (LinearGaussImproved.cpp:150-157)
     "no divergence test": [("const int cond1 = (res->x.vx + res->y.vy) * width < (-prm->delta_v) * cs;",
                             "const int cond1 = 0 * (width + cs) != 0;")],
