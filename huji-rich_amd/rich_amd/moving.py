@@ -34,14 +34,14 @@ from .gpu import RichGPU
 class GpuBackend:
     """The plugins on the GPU, one context per mesh."""
 
-    def __init__(self, gamma=5.0 / 3.0, cfl=0.3):
-        self.gamma, self.cfl_number = gamma, cfl
+    def __init__(self, gamma=5.0 / 3.0, cfl=0.3, order=2):
+        self.gamma, self.cfl_number, self.order = gamma, cfl, order
         self.g = None
 
     def set_mesh(self, mesh):
         if self.g is not None:
             self.g.close()
-        self.g = RichGPU(mesh, gamma=self.gamma, cfl=self.cfl_number, order=2)
+        self.g = RichGPU(mesh, gamma=self.gamma, cfl=self.cfl_number, order=self.order)
 
     def init_extensives(self, cells):
         self.g.set_cells(cells)
@@ -171,17 +171,36 @@ class HostMotion:
         return mesh.face_velocity(vx, vy)
 
 
+class LagrangianMotion:
+    """Lagrangian point motion (lagrangian.cpp: each point moves with its
+    cell's velocity; PointMotion::ApplyFix leaves the velocities,
+    point_motion.cpp:5-10) with StationaryBox face velocities."""
+
+    def point_velocity(self, mesh, cells):
+        return np.array(cells[2], dtype=np.float64), np.array(cells[3], dtype=np.float64)
+
+    def point_fix(self, mesh, vx, vy, dt, box):
+        return vx, vy
+
+    def face_velocity(self, mesh, vx, vy):
+        return mesh.face_velocity(vx, vy)
+
+
 class MovingMesh:
     """A moving-mesh run: points, cells (4, N) and extensives on the host
-    between steps; `step()` advances one TimeAdvance2Heun step.  `motion`
-    supplies the point and face velocities (default HostMotion)."""
+    between steps; `step()` advances one TimeAdvance2Heun step (integrator
+    2) or one TimeAdvance step (integrator 1).  `motion` supplies the point
+    and face velocities (default HostMotion)."""
 
     def __init__(self, x, y, cells, box, backend, gamma=5.0 / 3.0, motion=None,
-                 resort_every=25):
+                 resort_every=25, integrator=2):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.b, self.gamma = box, backend, gamma
         self.motion = motion if motion is not None else HostMotion(gamma)
+        if integrator not in (1, 2):
+            raise ValueError("integrator is 1 (TimeAdvance) or 2 (TimeAdvance2Heun)")
+        self.integrator = integrator
         self.resort_every, self.cycle, self.reorders = resort_every, 0, 0
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
@@ -200,6 +219,8 @@ class MovingMesh:
         return out
 
     def step(self):
+        if self.integrator == 1:
+            return self._step_euler()
         b, T = self.b, self.timing
         t = time.perf_counter()
         # Lagrangian + RoundCells point velocities, StationaryBox face velocities
@@ -251,6 +272,47 @@ class MovingMesh:
         ext[1] = 0.5 * (mid[1] + self.ext[1])
         ext[2] = (mid[2] + self.ext[2]) * 0.5
         ext[3] = (mid[3] + self.ext[3]) * 0.5
+        self.ext = ext
+        self.cells = self._plug(b.update_cells, ext)
+        self.time += dt
+        self.cycle += 1
+        return dt
+
+
+    def _step_euler(self):
+        """hdsim::TimeAdvance (hdsim2d.cpp:149-241): point and face
+        velocities, dt, ApplyFix, the fluxes and the extensive update on the
+        old mesh, ExternalForceContribution with ZeroForce, MoveMeshPoints
+        with the re-sort every resort_every cycles, the rebuild and the cell
+        update on the new mesh."""
+        b, T = self.b, self.timing
+        t = time.perf_counter()
+        mo = self.motion
+        vx, vy = mo.point_velocity(self.mesh, self.cells)
+        fw = mo.face_velocity(self.mesh, vx, vy)
+        T["motion_s"] += time.perf_counter() - t
+        dt = self._plug(b.cfl, self.cells, fw)
+        t = time.perf_counter()
+        vx, vy = mo.point_fix(self.mesh, vx, vy, dt, self.box)
+        fw = mo.face_velocity(self.mesh, vx, vy)
+        T["motion_s"] += time.perf_counter() - t
+        flux = self._plug(b.flux, self.cells, fw)
+        ext = self._plug(b.update_extensives, flux, dt, self.ext)
+        ext[0] += dt * 0.0
+        ext[2] += 0.0 * dt
+        ext[3] += 0.0 * dt
+        ext[1] += dt * 0.0
+        t = time.perf_counter()
+        self.x = self.x + dt * vx
+        self.y = self.y + dt * vy
+        order = self._resort_order()
+        if order is not None:
+            self.x, self.y = self.x[order], self.y[order]
+            self.cells, ext = self.cells[:, order], ext[:, order]
+            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
+        self.mesh = self.mesher.mesh(self.x, self.y)
+        T["mesh_s"] += time.perf_counter() - t
+        self._plug(b.set_mesh, self.mesh)
         self.ext = ext
         self.cells = self._plug(b.update_cells, ext)
         self.time += dt

@@ -211,3 +211,27 @@ def singular_cell_mesh(i=5, seed=9):
     tv = m.tess_volume.copy()
     tv[i] = lam
     return replace(m, tess_volume=tv, _handle=None), i, lam
+
+
+def triangle_cells(m):
+    """conservation_lagrangian's initial state (test.cpp:11-57): density 1;
+    pressure 2 and velocity (1, -1) for points inside the triangle
+    (0.5, 0.6), (0.7, 0.5), (0.4, 0.4) (boundary included), pressure 1 at
+    rest elsewhere."""
+    N = m.n_cells
+    x, y = m.pt_x[:N], m.pt_y[:N]
+    P = [(0.5, 0.6), (0.7, 0.5), (0.4, 0.4)]
+
+    def same_side(px, py, q, a, b):
+        cp1 = (b[0] - a[0]) * (py - a[1]) - (b[1] - a[1]) * (px - a[0])
+        cp2 = (b[0] - a[0]) * (q[1] - a[1]) - (b[1] - a[1]) * (q[0] - a[0])
+        return cp1 * cp2 >= 0
+
+    inside = (same_side(x, y, P[0], P[1], P[2]) & same_side(x, y, P[1], P[0], P[2])
+              & same_side(x, y, P[2], P[0], P[1]))
+    c = np.zeros((4, N))
+    c[0] = 1.0
+    c[1] = np.where(inside, 2.0, 1.0)
+    c[2] = np.where(inside, 1.0, 0.0)
+    c[3] = np.where(inside, -1.0, 0.0)
+    return c

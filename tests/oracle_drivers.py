@@ -11,8 +11,11 @@ import pyoracle as po
 class OracleBackend:
     """The plugins of the step on the oracle, one oracle per mesh."""
 
+    def __init__(self, order=2):
+        self.order = order
+
     def set_mesh(self, mesh):
-        self.o = po.Oracle(mesh, order=2)
+        self.o = po.Oracle(mesh, order=self.order)
 
     def init_extensives(self, cells):
         return self.o.init_extensives(cells)
@@ -46,6 +49,21 @@ class OracleMotion:
 
     def face_velocity(self, mesh, vx, vy):
         fw = po.Oracle(mesh, gamma=self.gamma).face_velocity(np.vstack([vx, vy]))
+        return fw[0], fw[1]
+
+
+class OracleLagrangianMotion:
+    """Lagrangian point motion (each point with its cell's velocity, no
+    fix) with the oracle's StationaryBox face velocities."""
+
+    def point_velocity(self, mesh, cells):
+        return np.array(cells[2], dtype=np.float64), np.array(cells[3], dtype=np.float64)
+
+    def point_fix(self, mesh, vx, vy, dt, box):
+        return vx, vy
+
+    def face_velocity(self, mesh, vx, vy):
+        fw = po.Oracle(mesh).face_velocity(np.vstack([vx, vy]))
         return fw[0], fw[1]
 
 

@@ -130,3 +130,30 @@ def test_resorted_moving_run_equals_oracle(device):
         run.close()
     else:
         run.b.close()
+
+
+def test_conservation_lagrangian_gpu_equals_oracle():
+    """The reference's conservation_lagrangian run (30 x 30 Cartesian grid,
+    Lagrangian points, SimpleFluxCalculator, TimeAdvance, to t = 0.05) with
+    the GPU plugins: every step equals the oracle-driven run bit for bit, and
+    the totals pass the reference's grader (equal at six significant
+    digits) with mass and energy within 1e-13."""
+    from rich_amd.moving import GpuBackend, LagrangianMotion
+    from oracle_drivers import OracleBackend, OracleLagrangianMotion
+    from test_conservation_lagrangian import check_totals, lagrangian_run
+    gpu = lagrangian_run(GpuBackend(order=1), LagrangianMotion())
+    ref = lagrangian_run(OracleBackend(order=1), OracleLagrangianMotion())
+    totals = [gpu.ext.sum(axis=1)]
+    step = 0
+    while ref.time < 0.05:
+        dt, dt_ref = gpu.step(), ref.step()
+        assert dt == dt_ref, step
+        np.testing.assert_array_equal(gpu.x, ref.x)
+        np.testing.assert_array_equal(gpu.y, ref.y)
+        np.testing.assert_array_equal(gpu.cells, ref.cells)
+        np.testing.assert_array_equal(gpu.ext, ref.ext)
+        totals.append(gpu.ext.sum(axis=1))
+        step += 1
+    assert step > 20 and gpu.reorders == ref.reorders >= 1
+    check_totals(totals)
+    gpu.b.close()
