@@ -112,7 +112,15 @@ __device__ __forceinline__ int ell_base(const DMesh& m, int i) {
 struct DParams {
   double gamma, cfl, delta_v, theta, delta_p;
   int slf;
+  // Walk the blocks in reverse order.  The step loop alternates the walk
+  // between consecutive kernels, so that each kernel starts on the data its
+  // predecessor wrote last, which the caches still hold.
+  int rev;
 };
+__device__ __forceinline__ int walk_block(const DParams& prm) {
+  return prm.rev ? static_cast<int>(gridDim.x) - 1 - static_cast<int>(blockIdx.x)
+                 : static_cast<int>(blockIdx.x);
+}
 // first-error record: atomicMin of (step, phase, index, code)
 struct DErr {
   unsigned long long* word;
@@ -557,7 +565,7 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
 
 __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
                                                 Rec* __restrict__ out, DParams prm, DErr err) {
-  slopes_cell(m, q, out, prm, err, blockIdx.x * blockDim.x + threadIdx.x);
+  slopes_cell(m, q, out, prm, err, walk_block(prm) * blockDim.x + threadIdx.x);
 }
 // Persistent form: a fixed grid walks the cells in tiles of blockDim.x.
 __global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict__ q,
@@ -671,7 +679,7 @@ __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q
                                                const unsigned long long* __restrict__ dt_slot) {
   const double dt =
       dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*dt_slot)) : 0.0;
-  flux2_face(m, q, rec, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x, dt);
+  flux2_face(m, q, rec, flux, prm, err, walk_block(prm) * blockDim.x + threadIdx.x, dt);
 }
 // ModularFluxCalculator over PCM; dt_slot as for k_flux2
 __global__ void __launch_bounds__(256) k_flux_pcm(DMesh m, const D4* __restrict__ q,
@@ -981,7 +989,7 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
 }
 
 __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm, DErr err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   const double dt = a.dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot))
                               : a.dt;
   if (i == 0) {

@@ -356,7 +356,12 @@ void drain_timing(rg_ctx* c) {
 
 DParams dparams(const rg_ctx* c) {
   return DParams{c->prm.gamma, c->prm.cfl, c->prm.delta_v, c->prm.theta, c->prm.delta_p,
-                 c->prm.slf};
+                 c->prm.slf, 0};
+}
+DParams dparams_walk(const rg_ctx* c, int rev) {
+  DParams p = dparams(c);
+  p.rev = rev;
+  return p;
 }
 
 DErr derr(rg_ctx* c, int step, int phase) { return DErr{c->words + 2, step, phase}; }
@@ -393,7 +398,8 @@ hipError_t read_one(rg_ctx* c, const T* src, size_t i, T* out) {
   return hipMemcpyAsync(out, src + i, sizeof(T), hipMemcpyDeviceToHost, c->stream);
 }
 
-hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out);
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out,
+                      int rev = 0);
 
 // The UniversalError entries of error e (see rg_error), recomputed from the
 // inputs the failing call read.
@@ -491,24 +497,27 @@ bool ready(const rg_ctx* c, bool need_cells, bool need_ext) {
   return c && c->has_mesh && (!need_cells || c->has_cells) && (!need_ext || c->has_ext);
 }
 
-hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out) {
+hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out, int rev) {
   Rec* r = out ? out : c->rec;
   return launch(c, KID_SLOPES, [&] {
-    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, r, dparams(c),
+    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, r, dparams_walk(c, rev),
                                                          derr(c, step, phase));
   });
 }
 
 // FluxCalculator on `cells` into c->flux (slopes into `rec`, default c->rec).
 // With a dt slot, the modular calculators write dt*area*F (for UPD_DELTA).
+// `rev` walks the slope kernel's blocks in reverse and the face kernel's
+// forward (0: the other way round).
 hipError_t run_flux(rg_ctx* c, const D4* cells, int step, int phase_base, Rec* rec = nullptr,
-                    const unsigned long long* dt_slot = nullptr) {
+                    const unsigned long long* dt_slot = nullptr, int rev = 0) {
   if (c->prm.order == 2) {
     Rec* r = rec ? rec : c->rec;
-    const hipError_t e = run_slopes(c, cells, step, phase_base, r);
+    const hipError_t e = run_slopes(c, cells, step, phase_base, r, rev);
     if (e != hipSuccess) return e;
     return launch(c, KID_FLUX2, [&] {
-      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, r, c->flux, dparams(c),
+      k_flux2<<<blocks_for(c->F), kBlock, 0, c->stream>>>(c->dm, cells, r, c->flux,
+                                                           dparams_walk(c, 1 - rev),
                                                            derr(c, step, phase_base + 1), dt_slot);
     });
   }
@@ -586,12 +595,16 @@ HeunBufs ctx_bufs(rg_ctx* c) {
 // mesh: the dt of slot s&1, fluxes, the update to the mid state, fluxes of
 // the mid state, 0.5*(mid + old) and the cell update with the next dt's
 // CFL terms into slot (s+1)&1.  dt_hist[hist] receives the step's dt.
+// Consecutive kernels walk their blocks in opposite directions (slopes,
+// faces, update: forward, reverse, forward in the first stage, reverse,
+// forward, reverse in the second), so each starts where the previous one
+// ended; the results do not depend on the order.
 hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs& h) {
   const int cur = s & 1, nxt = (s + 1) & 1;
   // the modular calculators write dt*area*F straight from the flux kernel
   const int delta = c->prm.order >= 2 ? UPD_DELTA : 0;
   const unsigned long long* slot = delta ? c->words + cur : nullptr;
-  hipError_t e = run_flux(c, h.P, s, 1, h.r1, slot);
+  hipError_t e = run_flux(c, h.P, s, 1, h.r1, slot, 0);
   if (e != hipSuccess) return e;
   UpdArgs a{};
   a.in = h.X;
@@ -606,7 +619,7 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
     k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, a, dparams(c), derr(c, s, 3));
   });
   if (e != hipSuccess) return e;
-  e = run_flux(c, h.Pm, s, 4, h.r2, slot);
+  e = run_flux(c, h.Pm, s, 4, h.r2, slot, 1);
   if (e != hipSuccess) return e;
   UpdArgs b{};
   b.in = h.Xm;
@@ -619,7 +632,8 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   b.step = hist;
   b.flags = UPD_FACES | delta | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   e = launch(c, KID_UPDATE, [&] {
-    k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, b, dparams(c), derr(c, s, 6));
+    k_update<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, b, dparams_walk(c, 1),
+                                                         derr(c, s, 6));
   });
   if (e != hipSuccess) return e;
   return run_reduce_min(c, nxt);
