@@ -2,8 +2,8 @@
 //
 // Per mesh (static geometry, computed once by k_geo_*):
 //   per face  FaceRec   {h = r_ij/2 seen from the first side, centroid}
-//             FaceFrame {centroid minus each side's CM, rotated-frame
-//                        normal n, face direction p}
+//             FaceFrame {centroid, rotated-frame normal n, face
+//                        direction p}
 //             area
 //   per cell  CellGeo   {inverse gradient matrix, width, kind, face count}
 //             -- the matrix of calc_naive_slope (or the least-squares sums
@@ -57,10 +57,11 @@ struct alignas(16) CellGeo {
 struct alignas(32) FaceRec {
   double hx, hy, cx, cy;
 };
-// the face centroid minus the CM of each side (first a, second b), the
-// normal n and the face direction p; |n| and |p| are recomputed with vabs
-struct alignas(64) FaceFrame {
-  double oxa, oya, oxb, oyb, nx, ny, px, py;
+// the face centroid, the normal n and the face direction p; |n| and |p| are
+// recomputed with vabs, and interp's centroid minus CM of each side with the
+// side's CM (the subtraction the reference evaluates)
+struct alignas(16) FaceFrame {
+  double cx, cy, nx, ny, px, py;
 };
 struct alignas(8) Inc {
   int nbr;
@@ -266,10 +267,7 @@ __global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceF
   const double s = qx * ppx + qy * ppy;
   const double pp = ppx * ppx + ppy * ppy;
   const double nx = qx - ppx * s / pp, ny = qy - ppy * s / pp;
-  // interp's centroid minus CM (LinearGaussImproved.cpp:159-172), for the
-  // ghost sides at the ghost's CM (:597-634)
-  const D2 ca = m.cm[a], cb = m.cm[b];
-  fframe[f] = FaceFrame{cx - ca.x, cy - ca.y, cx - cb.x, cy - cb.y, nx, ny, ppx, ppy};
+  fframe[f] = FaceFrame{cx, cy, nx, ny, ppx, ppy};
   // SlabSymmetry::calcArea (physical_geometry.cpp:8-11)
   area[f] = vabs(v1x - v2x, v1y - v2y);
 }
@@ -594,7 +592,9 @@ __device__ __forceinline__ Cell side_state(const DMesh& m, const D4* q, const Re
 
 // side_state of both sides of face f, with the loads of both sides issued
 // together: ghost ids are resolved first (a branch only waves with a ghost
-// face take), then both slope records and cells are loaded.
+// face take), then both slope records, cells and CMs are loaded.  interp's
+// centroid minus CM (LinearGaussImproved.cpp:159-172) takes a ghost side at
+// the ghost's CM (:597-634).
 __device__ __forceinline__ void face_states(const DMesh& m, const D4* q, const Rec* rec, int a,
                                             int b, const FaceFrame& fr, Cell& lf, Cell& rt) {
   const int id[2] = {a, b};
@@ -608,10 +608,12 @@ __device__ __forceinline__ void face_states(const DMesh& m, const D4* q, const R
   }
   Rec r[2];
   Cell v[2];
+  D2 cm[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
     r[u] = rec[src[u]];
     v[u] = ld_cell(q, src[u]);
+    cm[u] = m.cm[id[u]];
   }
   if (ghost && m.bc == BC_RIGID) {
     // RigidWallGenerator: reflected copy, zero gradient
@@ -627,8 +629,8 @@ __device__ __forceinline__ void face_states(const DMesh& m, const D4* q, const R
   }
   const Cell sxa{r[0].xd, r[0].xp, r[0].xvx, r[0].xvy}, sya{r[0].yd, r[0].yp, r[0].yvx, r[0].yvy};
   const Cell sxb{r[1].xd, r[1].xp, r[1].xvx, r[1].xvy}, syb{r[1].yd, r[1].yp, r[1].yvx, r[1].yvy};
-  lf = interp_off(v[0], sxa, sya, fr.oxa, fr.oya);
-  rt = interp_off(v[1], sxb, syb, fr.oxb, fr.oyb);
+  lf = interp_off(v[0], sxa, sya, fr.cx - cm[0].x, fr.cy - cm[0].y);
+  rt = interp_off(v[1], sxb, syb, fr.cx - cm[1].x, fr.cy - cm[1].y);
 }
 
 // ModularFluxCalculator (modular_flux_calculator.cpp:39-71) on one face,
@@ -704,8 +706,10 @@ __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restri
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceFrame fr = m.fframe[f];
-  const Cell lf = side_state(m, q, rec, m.ff[f], fr.oxa, fr.oya);
-  const Cell rt = side_state(m, q, rec, m.fs[f], fr.oxb, fr.oyb);
+  const int a = m.ff[f], b = m.fs[f];
+  const D2 ca = m.cm[a], cb = m.cm[b];
+  const Cell lf = side_state(m, q, rec, a, fr.cx - ca.x, fr.cy - ca.y);
+  const Cell rt = side_state(m, q, rec, b, fr.cx - cb.x, fr.cy - cb.y);
   left[f] = D4{lf.d, lf.p, lf.vx, lf.vy};
   right[f] = D4{rt.d, rt.p, rt.vx, rt.vy};
 }
