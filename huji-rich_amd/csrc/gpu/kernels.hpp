@@ -95,10 +95,16 @@ struct DMesh {
   const double* area;
   const double* width;  // GetWidth = sqrt(GetVolume/pi)
   // Incidences in wavefront-sliced ELL order: the cells of wavefront w are
-  // 64w..64w+63; slot s (the s-th face in ascending order) of lane l is entry
-  // wofs[w] + 64 s + l.  One slot load across a wavefront reads contiguous
-  // memory, where a CSR walk reads one scattered line per lane.
+  // 64w..64w+63; slot s (the s-th face in ascending order) of the cell in
+  // column l is entry wofs[w] + 64 s + l.  One slot load across a wavefront
+  // reads contiguous memory, where a CSR walk reads one scattered line per
+  // lane.  Within a wavefront the columns hold the cells in descending order
+  // of face count (lanepos: a cell's column; lanecell: the cell of a column,
+  // as an offset from 64w), so that the entries a slot load needs form a
+  // prefix of the slot and no memory sector holds only padding.
   const int* wofs;
+  const unsigned char* lanepos;   // [N]
+  const unsigned char* lanecell;  // [64 * waves]
   const int* enbr;       // neighbour id
   const unsigned* einfo; // face | INFO_SECOND | INFO_MIRROR
   const D2* eh;          // r_ij/2 as seen from the cell (negated on the second side)
@@ -107,7 +113,7 @@ struct DMesh {
 };
 
 __device__ __forceinline__ int ell_base(const DMesh& m, int i) {
-  return m.wofs[i >> 6] + (i & 63);
+  return m.wofs[i >> 6] + m.lanepos[i];
 }
 
 struct DParams {
@@ -561,9 +567,13 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
   out[i] = Rec{sx.d, sx.p, sx.vx, sx.vy, sy.d, sy.p, sy.vx, sy.vy};
 }
 
+// Lane l of wavefront w takes the cell of ELL column l, so that the lanes
+// that run out of faces first are the last ones.
 __global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
                                                 Rec* __restrict__ out, DParams prm, DErr err) {
-  slopes_cell(m, q, out, prm, err, walk_block(prm) * blockDim.x + threadIdx.x);
+  const int t = walk_block(prm) * blockDim.x + threadIdx.x;
+  if (t >= m.N) return;
+  slopes_cell(m, q, out, prm, err, (t & ~63) + m.lanecell[t]);
 }
 // Persistent form: a fixed grid walks the cells in tiles of blockDim.x.
 __global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict__ q,

@@ -41,6 +41,7 @@ struct MTables {
   const FaceM* face;
   const CellGeoM* cgeo;
   const int* wofs;
+  const unsigned char* lanepos;  // ELL column of each cell (as in DMesh)
   const int* enbr;       // neighbour id (ghost ids as in the mesh)
   const float2* eh;      // r_ij/2 seen from the cell (gradient pass)
   const float2* eo;      // centroid - cell CM (limiter pass)
@@ -48,7 +49,7 @@ struct MTables {
 };
 
 __device__ __forceinline__ int ell_base(const MTables& t, int i) {
-  return t.wofs[i >> 6] + (i & 63);
+  return t.wofs[i >> 6] + t.lanepos[i];
 }
 
 struct CellF {
@@ -170,13 +171,12 @@ __global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
 }
 
 // ELL geometry of every incidence: one thread per cell over its faces.
-__global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, const int* __restrict__ wofs,
-                                                       float2* __restrict__ eh,
+__global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, float2* __restrict__ eh,
                                                        float2* __restrict__ eo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const D2 ci = m.cm[i];
-  const int base = wofs[i >> 6] + (i & 63);
+  const int base = ell_base(m, i);
   for (int k = m.off[i]; k < m.off[i + 1]; ++k) {
     const unsigned info = m.inc[k].info;
     const FaceRec r = m.frec[info & INFO_FACE];

@@ -183,6 +183,7 @@ struct rg_ctx {
     std::vector<int> ff, fs;     // [F]
     std::vector<int> gsrc;       // [G]
     std::vector<int> wofs;       // [waves+1]
+    std::vector<unsigned char> lanepos;  // [N] ELL column of each cell
   } adj_src;
 
   // moving mesh: point velocities kept across rg_replace_mesh, and where
@@ -722,7 +723,7 @@ hipError_t build_adjoint_tables(rg_ctx* c) {
   const int N = c->N, F = c->F;
   const size_t ne = static_cast<size_t>(c->dm_ne);
   auto ell_at = [&](int i, int k) {
-    return static_cast<size_t>(a.wofs[i / 64] + i % 64) +
+    return static_cast<size_t>(a.wofs[i / 64] + a.lanepos[static_cast<size_t>(i)]) +
            64 * static_cast<size_t>(k - a.csr_off[i]);
   };
   // ELL entry of each face on its first [2f] and second [2f + 1] side
@@ -1146,28 +1147,40 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, upload(c, &gsrc_d, gsrc.data(), gsrc.size()));
   HIPCHK(c, upload(c, &gn_d, gn.data(), gn.size()));
   // wavefront-sliced ELL of the incidences (slot s of a cell = its s-th
-  // face in ascending order)
+  // face in ascending order); within each wavefront the columns hold the
+  // cells by descending face count (ties in cell order)
   const int nw = (N + 63) / 64;
   std::vector<int> wofs(static_cast<size_t>(nw) + 1, 0);
+  std::vector<unsigned char> lanepos(static_cast<size_t>(std::max(N, 1)), 0);
+  std::vector<unsigned char> lanecell(64 * static_cast<size_t>(std::max(nw, 1)), 0);
   for (int w = 0; w < nw; ++w) {
-    int wd = 0;
-    for (int i = 64 * w; i < std::min(N, 64 * w + 64); ++i)
-      wd = std::max(wd, m->csr_off[i + 1] - m->csr_off[i]);
-    wofs[w + 1] = wofs[w] + 64 * wd;
+    const int i0 = 64 * w, n = std::min(N, i0 + 64) - i0;
+    int col[64];
+    for (int l = 0; l < 64; ++l) col[l] = l;
+    auto deg = [&](int l) { return m->csr_off[i0 + l + 1] - m->csr_off[i0 + l]; };
+    std::stable_sort(col, col + n, [&](int x, int y) { return deg(x) > deg(y); });
+    for (int l = 0; l < 64; ++l) {
+      lanecell[64 * static_cast<size_t>(w) + l] = static_cast<unsigned char>(col[l]);
+      if (l < n) lanepos[static_cast<size_t>(i0 + col[l])] = static_cast<unsigned char>(l);
+    }
+    wofs[w + 1] = wofs[w] + 64 * (n > 0 ? deg(col[0]) : 0);
   }
   const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
   c->dm_ne = static_cast<int>(ne);
   auto ell_at = [&](int i, int k) {
-    return static_cast<size_t>(wofs[i / 64] + i % 64) +
+    return static_cast<size_t>(wofs[i / 64] + lanepos[static_cast<size_t>(i)]) +
            64 * static_cast<size_t>(k - m->csr_off[i]);
   };
   // the ELL arrays are filled on the device from the incidences (k_geo_ell);
   // padding entries stay zero
   int *wofs_d, *enbr_d;
+  unsigned char *lanepos_d, *lanecell_d;
   unsigned* einfo_d;
   D2 *eh, *eo;
   double* earea;
   HIPCHK(c, upload(c, &wofs_d, wofs.data(), wofs.size()));
+  HIPCHK(c, upload(c, &lanepos_d, lanepos.data(), lanepos.size()));
+  HIPCHK(c, upload(c, &lanecell_d, lanecell.data(), lanecell.size()));
   HIPCHK(c, dalloc(c, &enbr_d, ne));
   HIPCHK(c, dalloc(c, &einfo_d, ne));
   HIPCHK(c, hipMemsetAsync(enbr_d, 0, sizeof(int) * ne, c->stream));
@@ -1206,6 +1219,8 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   d.area = area;
   d.width = width;
   d.wofs = wofs_d;
+  d.lanepos = lanepos_d;
+  d.lanecell = lanecell_d;
   d.enbr = enbr_d;
   d.einfo = einfo_d;
   d.eh = eh;
@@ -1311,10 +1326,10 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
       k_geo_mixed_cells<<<blocks_for(N), kBlock, 0, c->stream>>>(d, cgeoM);
     }));
     HIPCHK(c, launch(c, KID_GEO, [&] {
-      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, wofs_d, deh, deo);
+      k_geo_mixed_ell<<<blocks_for(N), kBlock, 0, c->stream>>>(d, deh, deo);
     }));
     HIPCHK(c, hipStreamSynchronize(c->stream));  // host vectors above are uploaded
-    c->mt = MTables{faceM, cgeoM, wofs_d, enbr_d, deh, deo, deinfo};
+    c->mt = MTables{faceM, cgeoM, wofs_d, lanepos_d, enbr_d, deh, deo, deinfo};
   }
   c->dm = d;
   c->N = N;
@@ -1327,6 +1342,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   c->adj_src.fs.assign(m->face_second, m->face_second + F);
   c->adj_src.gsrc.assign(gsrc.begin(), gsrc.begin() + G);
   c->adj_src.wofs = std::move(wofs);
+  c->adj_src.lanepos = std::move(lanepos);
   c->has_mesh = true;
   drain_pool(c);
   return RG_OK;
