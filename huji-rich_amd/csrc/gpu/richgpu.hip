@@ -1153,18 +1153,36 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   std::vector<int> wofs(static_cast<size_t>(nw) + 1, 0);
   std::vector<unsigned char> lanepos(static_cast<size_t>(std::max(N, 1)), 0);
   std::vector<unsigned char> lanecell(64 * static_cast<size_t>(std::max(nw, 1)), 0);
-  for (int w = 0; w < nw; ++w) {
-    const int i0 = 64 * w, n = std::min(N, i0 + 64) - i0;
-    int col[64];
-    for (int l = 0; l < 64; ++l) col[l] = l;
-    auto deg = [&](int l) { return m->csr_off[i0 + l + 1] - m->csr_off[i0 + l]; };
-    std::stable_sort(col, col + n, [&](int x, int y) { return deg(x) > deg(y); });
-    for (int l = 0; l < 64; ++l) {
-      lanecell[64 * static_cast<size_t>(w) + l] = static_cast<unsigned char>(col[l]);
-      if (l < n) lanepos[static_cast<size_t>(i0 + col[l])] = static_cast<unsigned char>(l);
+  rmp::parallel_for(nw, [&](int64_t ww) {  // a counting sort per wavefront
+    const int w = static_cast<int>(ww), i0 = 64 * w, n = std::min(N, i0 + 64) - i0;
+    int dg[64], wd = 0;
+    for (int l = 0; l < n; ++l) {
+      dg[l] = m->csr_off[i0 + l + 1] - m->csr_off[i0 + l];
+      wd = std::max(wd, dg[l]);
     }
-    wofs[w + 1] = wofs[w] + 64 * (n > 0 ? deg(col[0]) : 0);
-  }
+    int small[65] = {0};
+    std::vector<int> large;  // face counts above 63 (not met in Voronoi meshes of interest)
+    int* start = small;
+    if (wd > 63) {
+      large.assign(static_cast<size_t>(wd) + 1, 0);
+      start = large.data();
+    }
+    for (int l = 0; l < n; ++l) ++start[wd - dg[l]];
+    for (int k = 0, acc = 0; k <= wd; ++k) {
+      const int cnt = start[k];
+      start[k] = acc;
+      acc += cnt;
+    }
+    unsigned char* cells = lanecell.data() + 64 * static_cast<size_t>(w);
+    for (int l = 0; l < n; ++l) {
+      const int col = start[wd - dg[l]]++;
+      lanepos[static_cast<size_t>(i0 + l)] = static_cast<unsigned char>(col);
+      cells[col] = static_cast<unsigned char>(l);
+    }
+    for (int l = n; l < 64; ++l) cells[l] = static_cast<unsigned char>(l);
+    wofs[static_cast<size_t>(w) + 1] = 64 * wd;
+  }, 256);
+  for (int w = 0; w < nw; ++w) wofs[w + 1] += wofs[w];
   const size_t ne = static_cast<size_t>(std::max(wofs[nw], 1));
   c->dm_ne = static_cast<int>(ne);
   auto ell_at = [&](int i, int k) {

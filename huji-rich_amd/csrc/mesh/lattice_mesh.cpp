@@ -915,6 +915,32 @@ extern "C" rm_mesh* rm_mover_mesh(rm_mover* mv, const double* x, const double* y
                                mv->x1, mv->y0, mv->y1);
 }
 
+extern "C" int rm_mover_permute(rm_mover* mv, const int32_t* order) {
+  if (!mv || !order) return -1;
+  const int32_t N = mv->N;
+  std::vector<int32_t> inv(static_cast<size_t>(N), -1);
+  for (int32_t i = 0; i < N; ++i) {
+    const int32_t o = order[i];
+    if (o < 0 || o >= N || inv[static_cast<size_t>(o)] >= 0) return -1;
+    inv[static_cast<size_t>(o)] = i;
+  }
+  if (mv->tri.empty()) return 0;  // nothing kept: the next mesh is built afresh
+  // the same triangles, points and ghosts under the new point numbers
+  for (auto& t : mv->tri)
+    if (t >= 0 && t < N) t = inv[static_cast<size_t>(t)];
+  for (auto& g : mv->go) g = inv[static_cast<size_t>(g)];
+  std::vector<uint8_t> has(mv->has.size());
+  std::vector<double> X(mv->X.begin(), mv->X.begin() + N), Y(mv->Y.begin(), mv->Y.begin() + N);
+  for (int32_t i = 0; i < N; ++i) {
+    const size_t o = static_cast<size_t>(order[i]);
+    for (size_t w = 0; w < 4; ++w) has[w * N + i] = mv->has[w * N + o];
+    mv->X[static_cast<size_t>(i)] = X[o];
+    mv->Y[static_cast<size_t>(i)] = Y[o];
+  }
+  mv->has.swap(has);
+  return 0;
+}
+
 extern "C" void rm_mover_counts(const rm_mover* mv, int32_t* builds, int32_t* repairs) {
   if (builds) *builds = mv ? mv->builds : 0;
   if (repairs) *repairs = mv ? mv->repairs : 0;

@@ -54,6 +54,8 @@ def _lib():
         lib.rm_mover_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
                                         ctypes.POINTER(ctypes.c_int32)]
         lib.rm_mover_free.argtypes = [ctypes.c_void_p]
+        lib.rm_mover_permute.restype = ctypes.c_int
+        lib.rm_mover_permute.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lib.rm_box_points_build.restype = ctypes.c_void_p
         lib.rm_box_points_build.argtypes = [ctypes.POINTER(_BoxPointsParams)]
         lib.rm_delaunay.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
@@ -195,6 +197,14 @@ class BoxMesher:
         if not h:
             raise ValueError("rm_mover_mesh could not triangulate the points")
         return _export(h, RIGID, lattice_ij=False)
+
+    def permute(self, order):
+        """Renumber the points, new point i = old point order[i] (rm_mover_permute):
+        the kept triangulation follows, so the next mesh is repaired, not
+        built afresh."""
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        if o.shape != (self.n,) or self._lib.rm_mover_permute(self._h, _ffi.ptr(o)) != 0:
+            raise ValueError("expected a permutation of the points")
 
     def counts(self):
         """(full triangulations, flip repairs) so far."""

@@ -251,7 +251,7 @@ class MovingMesh:
         if order is not None:
             self.x, self.y, vx, vy = self.x[order], self.y[order], vx[order], vy[order]
             self.cells, self.ext, mid = self.cells[:, order], self.ext[:, order], mid[:, order]
-            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
+            self.mesher.permute(order)
         self.mesh = self.mesher.mesh(self.x, self.y)
         T["mesh_s"] += time.perf_counter() - t
         self._plug(b.set_mesh, self.mesh)
@@ -309,7 +309,7 @@ class MovingMesh:
         if order is not None:
             self.x, self.y = self.x[order], self.y[order]
             self.cells, ext = self.cells[:, order], ext[:, order]
-            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
+            self.mesher.permute(order)
         self.mesh = self.mesher.mesh(self.x, self.y)
         T["mesh_s"] += time.perf_counter() - t
         self._plug(b.set_mesh, self.mesh)
@@ -357,7 +357,7 @@ class DeviceMovingMesh:
         if order is not None:
             self.x, self.y = self.x[order], self.y[order]
             self.g.permute_cells(order)
-            self.mesher = rmesh.BoxMesher(len(self.x), self.box)
+            self.mesher.permute(order)
         self.mesh = self.mesher.mesh(self.x, self.y)
         t2 = time.perf_counter()
         self.g.replace_mesh(self.mesh)

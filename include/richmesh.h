@@ -104,6 +104,13 @@ rm_mesh* rm_box_points_build(const rm_box_points_params* p);
 typedef struct rm_mover rm_mover;
 rm_mover* rm_mover_create(const rm_box_points_params* p);
 rm_mesh* rm_mover_mesh(rm_mover* mv, const double* x, const double* y);
+/* The points renumbered, new point i = old point order[i] (a permutation of
+ * [0, n), e.g. the Hilbert re-sort MoveMeshPoints applies every 25 cycles,
+ * hdsim2d.cpp:222-228): the kept triangulation follows the new numbers, so
+ * the next rm_mover_mesh repairs it as usual instead of triangulating
+ * afresh.  Returns 0, or -1 for anything but a permutation (nothing
+ * changed). */
+int rm_mover_permute(rm_mover* mv, const int32_t* order);
 /* Full triangulations and flip repairs so far. */
 void rm_mover_counts(const rm_mover* mv, int32_t* builds, int32_t* repairs);
 void rm_mover_free(rm_mover* mv);

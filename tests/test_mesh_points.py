@@ -239,3 +239,49 @@ def test_mesher_threads_do_not_change_the_mesh(tmp_path):
         res[t] = np.load(path)
     for k in res["1"].files:
         np.testing.assert_array_equal(res["1"][k], res["8"][k], err_msg=k)
+
+
+def _same_geometry(a, b):
+    """The same Voronoi mesh up to the numbering of ghosts: faces between
+    real cells, each real cell's volumes and CM, and the ghosts as
+    (original, position)."""
+    N = a.n_cells
+    assert (b.n_cells, b.n_faces, b.n_ghosts) == (N, a.n_faces, a.n_ghosts)
+
+    def real_pairs(m):
+        return {tuple(sorted(p)) for p in zip(m.face_first, m.face_second) if max(p) < N}
+
+    def ghosts(m):
+        return sorted((int(o), round(float(x), 12), round(float(y), 12)) for o, x, y in
+                      zip(m.ghost_orig, m.pt_x[N:], m.pt_y[N:]))
+
+    assert real_pairs(a) == real_pairs(b)
+    assert ghosts(a) == ghosts(b)
+    np.testing.assert_allclose(a.volume, b.volume, rtol=1e-12)
+    np.testing.assert_allclose(a.tess_volume, b.tess_volume, rtol=1e-12)
+    np.testing.assert_allclose(a.cm_x[:N], b.cm_x[:N], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(a.cm_y[:N], b.cm_y[:N], rtol=0, atol=1e-13)
+
+
+def test_box_mesher_follows_a_renumbering():
+    """rm_mover_permute: after the points are renumbered the kept
+    triangulation is repaired (no fresh build) into the mesh a fresh build
+    of the renumbered points gives; anything but a permutation is refused."""
+    m0 = rmesh.lattice(30, 30, BOX, jitter=1e-3, seed=4)
+    N = m0.n_cells
+    rng = np.random.default_rng(3)
+    x, y = m0.pt_x[:N].copy(), m0.pt_y[:N].copy()
+    B = rmesh.BoxMesher(N, BOX)
+    B.mesh(x, y)
+    order = rng.permutation(N)
+    B.permute(order)
+    x, y = x[order], y[order]
+    x = np.clip(x + 0.03 * (2 / 30) * rng.standard_normal(N), -0.999, 0.999)
+    y = np.clip(y + 0.03 * (2 / 30) * rng.standard_normal(N), -0.999, 0.999)
+    _same_geometry(B.mesh(x, y), rmesh.points(x, y, BOX))
+    assert B.counts() == (1, 1)
+    bad = order.copy()
+    bad[0] = bad[1]
+    with pytest.raises(ValueError):
+        B.permute(bad)
+    B.close()
