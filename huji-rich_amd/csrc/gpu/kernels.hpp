@@ -908,7 +908,11 @@ struct UpdArgs {
 // (hydrodynamics_2d.cpp:348-372), average_extensive (hdsim2d.cpp:311-320),
 // regular_update (simple_cell_updater.cpp:22-59), TimeStepCalculator.
 // One cell with a given dt; returns its CFL inverse time step when
-// UPD_CFL is set (else +inf).
+// UPD_CFL is set (else +inf).  kEarly issues the loads that do not depend on
+// the face sum (Heun operand, volume, width, error word) before it, so that
+// they do not wait behind the stores (more registers: the one-cell-per-thread
+// kernel only).
+template <bool kEarly = false>
 __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
                                               const DParams& prm, const DErr& err, int i,
                                               double dt) {
@@ -916,7 +920,19 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
   // After a device-detected error the updates stop: the state stays as the
   // failing call left it (the reference throws there), so that the error's
   // entries can be recomputed from it.
-  if (*err.word != ~0ull) return cfl_v;
+  bool frozen = false;
+  D4 o{0, 0, 0, 0};
+  double vol = 0, wid = 0;
+  if (kEarly) {
+    if (a.flags & UPD_AVERAGE) o = a.avg[i];
+    if (a.flags & UPD_CELLS) {
+      vol = m.vol[i];
+      if (a.flags & UPD_CFL) wid = m.width[i];
+    }
+    frozen = *err.word != ~0ull;
+  } else if (*err.word != ~0ull) {
+    return cfl_v;
+  }
   {
     const D4 x = a.in[i];
     double em = x.a, eE = x.b, ex = x.c, ey = x.d;
@@ -979,15 +995,16 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
       eE += dt * 0.0;
     }
     if (a.flags & UPD_AVERAGE) {
-      const D4 o = a.avg[i];
+      if (!kEarly) o = a.avg[i];
       em = 0.5 * (em + o.a);
       eE = 0.5 * (eE + o.b);
       ex = (ex + o.c) * 0.5;
       ey = (ey + o.d) * 0.5;
     }
+    if (kEarly && frozen) return cfl_v;
     a.out[i] = D4{em, eE, ex, ey};
     if (a.flags & UPD_CELLS) {
-      const double d = em / m.vol[i];
+      const double d = em / (kEarly ? vol : m.vol[i]);
       const double vx = ex / em, vy = ey / em;
       const double e = eE / em - 0.5 * (vx * vx + vy * vy);
       if (d < 0)
@@ -996,7 +1013,8 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
         report(err, i, ERR_NEGATIVE_THERMAL);
       const double p = (prm.gamma - 1) * e * d;
       a.cells[i] = D4{d, p, vx, vy};
-      if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, m.width[i], Cell{d, p, vx, vy});
+      if (a.flags & UPD_CFL)
+        cfl_v = cfl_inv(m, i, prm.gamma, kEarly ? wid : m.width[i], Cell{d, p, vx, vy});
     }
   }
   return cfl_v;
@@ -1011,7 +1029,7 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
     if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
   }
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
-  if (i < m.N) cfl_v = update_cell(m, a, prm, err, i, dt);
+  if (i < m.N) cfl_v = update_cell<true>(m, a, prm, err, i, dt);
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
 }
 
