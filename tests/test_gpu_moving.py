@@ -7,7 +7,7 @@ import pytest
 
 import cases
 import pyoracle as po
-from oracle_drivers import oracle_moving_mesh
+from oracle_drivers import OracleBackend, oracle_moving_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -156,4 +156,32 @@ def test_conservation_lagrangian_gpu_equals_oracle():
         step += 1
     assert step > 20 and gpu.reorders == ref.reorders >= 1
     check_totals(totals)
+    gpu.b.close()
+
+
+def test_riemann_messy_grid_gpu_equals_oracle():
+    """The reference's riemann_profiles_messy_grid run (Lagrangian +
+    RoundCells points, SimpleFluxCalculator, TimeAdvance, to t = 0.225)
+    with the GPU plugins: every step equals the oracle-driven run bit for
+    bit, and the grader's goodness of fit passes."""
+    from rich_amd.moving import GpuBackend, HostMotion
+    from test_riemann_messy_grid import (REFERENCE_FIX_BOX, OracleMotionReferenceFix, grade,
+                                         messy_grid_run)
+
+    class HostMotionReferenceFix(HostMotion):
+        def point_fix(self, mesh, vx, vy, dt, box):
+            return super().point_fix(mesh, vx, vy, dt, REFERENCE_FIX_BOX)
+
+    gpu = messy_grid_run(GpuBackend(order=1), HostMotionReferenceFix())
+    ref = messy_grid_run(OracleBackend(order=1), OracleMotionReferenceFix())
+    step = 0
+    while ref.time < 0.225:
+        dt, dt_ref = gpu.step(), ref.step()
+        assert dt == dt_ref, step
+        np.testing.assert_array_equal(gpu.x, ref.x)
+        np.testing.assert_array_equal(gpu.y, ref.y)
+        np.testing.assert_array_equal(gpu.cells, ref.cells)
+        step += 1
+    gd, gp, gv = grade(gpu)
+    assert gd < 0.1 and gp < 0.11 and gv < 0.11
     gpu.b.close()
