@@ -206,7 +206,7 @@ constexpr int CH_M = 8;
 __global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const F4* __restrict__ q,
                                                       RecM* __restrict__ out, DParams prm,
                                                       DErr err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const int b = m.off[i], e = m.off[i + 1], deg = e - b;
   const int eb = ell_base(t, i);
@@ -416,7 +416,7 @@ __device__ __forceinline__ int hllc_face_m(const CellF& L, const CellF& R, float
 __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4* __restrict__ q,
                                                     const RecM* __restrict__ rec,
                                                     F4* __restrict__ flux, DParams prm, DErr err) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = walk_block(prm) * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceM fr = t.face[f];
   CellF lt, rt;
@@ -463,7 +463,7 @@ struct UpdArgsM {
 
 __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArgsM a, DParams prm,
                                                       DErr err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
   if (i == 0) {
     if (a.dt_hist) a.dt_hist[a.step] = dt;

@@ -793,14 +793,17 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
   for (int s = 0; s < nsteps; ++s) {
     const int cur = s & 1, nxt = (s + 1) & 1;
     for (int stage = 0; stage < 2; ++stage) {
+      // consecutive kernels walk their blocks in opposite directions, as in
+      // the FP64 step (heun_step)
       const F4* q = stage == 0 ? c->primM : c->mid_primM;
       HIPCHK(c, launch(c, KID_SLOPES_MIXED, [&] {
-        k_slopes_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, q, c->recM,
-                                                                dparams(c), derr(c, s, 1 + 3 * stage));
+        k_slopes_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(
+            c->dm, c->mt, q, c->recM, dparams_walk(c, stage), derr(c, s, 1 + 3 * stage));
       }));
       HIPCHK(c, launch(c, KID_FLUX_MIXED, [&] {
-        k_flux_mixed<<<blocks_for(F), kBlock, 0, c->stream>>>(c->dm, c->mt, q, c->recM, c->fluxM,
-                                                              dparams(c), derr(c, s, 2 + 3 * stage));
+        k_flux_mixed<<<blocks_for(F), kBlock, 0, c->stream>>>(
+            c->dm, c->mt, q, c->recM, c->fluxM, dparams_walk(c, 1 - stage),
+            derr(c, s, 2 + 3 * stage));
       }));
       UpdArgsM a{};
       a.flux = c->fluxM;
@@ -822,7 +825,8 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
         a.flags = UPD_AVERAGE | UPD_CFL;
       }
       HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
-        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, a, dparams(c),
+        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, a,
+                                                                dparams_walk(c, stage),
                                                                 derr(c, s, 3 + 3 * stage));
       }));
     }
