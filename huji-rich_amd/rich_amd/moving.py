@@ -322,15 +322,19 @@ class MovingMesh:
 
 class DeviceMovingMesh:
     """A moving-mesh run with the state on the GPU; `step()` advances one
-    TimeAdvance2Heun step, `cells()` reads the cells back."""
+    TimeAdvance2Heun step, `cells()` reads the cells back.  `fix_box` is the
+    box of RoundCells' overshoot fix (default: the mesh's box, RoundCells
+    built with the outer boundary; RoundCells built without one fixes
+    against [-1, 1]^2, round_cells.cpp:10-11)."""
 
     _resort_order = _resort_order
 
     def __init__(self, x, y, cells, box, gamma=5.0 / 3.0, cfl=0.3, chi=0.15, eta=0.02,
-                 device_geometry=True, resort_every=25):
+                 device_geometry=True, resort_every=25, fix_box=None):
         self.x = np.array(x, dtype=np.float64)
         self.y = np.array(y, dtype=np.float64)
         self.box, self.chi, self.eta = tuple(float(v) for v in box), chi, eta
+        self.fix_box = self.box if fix_box is None else tuple(float(v) for v in fix_box)
         self.resort_every, self.cycle, self.reorders = resort_every, 0, 0
         self.mesher = rmesh.BoxMesher(len(self.x), box)
         self.mesh = self.mesher.mesh(self.x, self.y)
@@ -346,7 +350,7 @@ class DeviceMovingMesh:
     def step(self):
         T = self.timing
         t0 = time.perf_counter()
-        dt, vx, vy = self.g.moving_begin(self.chi, self.eta, self.box)
+        dt, vx, vy = self.g.moving_begin(self.chi, self.eta, self.fix_box)
         t1 = time.perf_counter()
         # MoveMeshPoints (hydrodynamics_2d.cpp:218-247), the Hilbert re-sort
         # every resort_every cycles (the state follows on the device), and

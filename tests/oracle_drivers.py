@@ -11,11 +11,11 @@ import pyoracle as po
 class OracleBackend:
     """The plugins of the step on the oracle, one oracle per mesh."""
 
-    def __init__(self, order=2):
-        self.order = order
+    def __init__(self, order=2, gamma=5.0 / 3.0):
+        self.order, self.gamma = order, gamma
 
     def set_mesh(self, mesh):
-        self.o = po.Oracle(mesh, order=self.order)
+        self.o = po.Oracle(mesh, order=self.order, gamma=self.gamma)
 
     def init_extensives(self, cells):
         return self.o.init_extensives(cells)
