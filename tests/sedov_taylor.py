@@ -48,10 +48,11 @@ def profiles(V, g, j, w=0.0):
     return lam, f, gd, h
 
 
-def tables(g, j, n=4000):
-    """Profiles sampled from just off the centre to the shock, by increasing r."""
+def tables(g, j, n=4000, offset=None):
+    """Profiles at n values of V spaced evenly from V0 + offset (default
+    1e-9 of the range) to the shock, by increasing r."""
     V0, V2 = v_range(g, j)
-    V = np.linspace(V0 + 1e-9 * (V2 - V0), V2, n)
+    V = np.linspace(V0 + (1e-9 * (V2 - V0) if offset is None else offset), V2, n)
     return profiles(V, g, j)
 
 

@@ -202,3 +202,27 @@ def test_acoustic_wave_equals_oracle():
     np.testing.assert_array_equal(gc, cells)
     assert all(e < 5e-8 for e in error_norms(m, gc, float(np.sum(gdts))))
     g.close()
+
+
+def test_eulerian_sedov_taylor_equals_oracle():
+    """The reference's eulerian_sedov_taylor run (50 x 50 rigid grid, first
+    order, 100 TimeAdvance steps) through rg_advance: dts and cells equal
+    the oracle's bit for bit, and the grader's comparison with the
+    Sedov-Taylor profiles passes."""
+    from rich_amd import mesh as rmesh
+    from rich_amd.gpu import RichGPU
+    from test_sedov_taylor_grid import grade_cells, sedov_grid_cells
+    m = rmesh.lattice(50, 50, (0.0, 1.0, 0.0, 1.0), jitter=0.0, seed=1, bc=rmesh.RIGID)
+    N = m.n_cells
+    c = sedov_grid_cells(m)
+    o = po.Oracle(m, order=1)
+    co, _, do = o.run(c, o.init_extensives(c), 100, integrator=1)
+    g = RichGPU(m, order=1)
+    g.set_cells(c)
+    g.init_extensives()
+    np.testing.assert_array_equal(g.advance(100, integrator=1), do)
+    gc = g.get_cells()
+    np.testing.assert_array_equal(gc, co)
+    _, (ld, lp, lv) = grade_cells(m.pt_x[:N], m.pt_y[:N], gc)
+    assert ld < 0.5 and lp < 15 and lv < 1.5
+    g.close()

@@ -40,11 +40,15 @@ def goodness_of_fit(sim, exact):
 
 
 def grade(run):
-    ids = np.argsort(run.x)
-    xs = run.x[ids]
-    d, u, p = exact_riemann.sample(xs, run.time, 0.5, (1.0, 0.0, 2.0), (1.0, 0.0, 1.0), GAMMA)
-    return (goodness_of_fit(run.cells[0][ids], d), goodness_of_fit(run.cells[1][ids], p),
-            goodness_of_fit(run.cells[2][ids], u))
+    return grade_state(run.x, run.time, run.cells)
+
+
+def grade_state(x, t, cells):
+    ids = np.argsort(x)
+    xs = x[ids]
+    d, u, p = exact_riemann.sample(xs, t, 0.5, (1.0, 0.0, 2.0), (1.0, 0.0, 1.0), GAMMA)
+    return (goodness_of_fit(cells[0][ids], d), goodness_of_fit(cells[1][ids], p),
+            goodness_of_fit(cells[2][ids], u))
 
 
 def test_riemann_messy_grid_oracle():
@@ -58,3 +62,21 @@ def test_riemann_messy_grid_oracle():
     gx = (np.floor(run.x * 30) + 0.5) / 30
     gy = (np.floor(run.y * 30) + 0.5) / 30
     assert np.hypot(run.x - gx, run.y - gy).max() > 0.005
+
+
+def test_riemann_messy_grid_second_order_oracle():
+    """The reference's riemann_profiles_messy_grid_2 run: the same problem
+    at second order (LinearGaussImproved with RigidWallGenerator,
+    ModularFluxCalculator, TimeAdvance2Heun); bounds 0.067, 0.053 and
+    0.062."""
+    m = rmesh.lattice(30, 30, (0.0, 1.0, 0.0, 1.0), jitter=0.0, seed=1, bc=rmesh.RIGID)
+    N = m.n_cells
+    x = m.pt_x[:N]
+    cells = np.vstack([np.ones(N), np.where(x < 0.5, 2.0, 1.0), np.zeros(N), np.zeros(N)])
+    run = MovingMesh(m.pt_x[:N], m.pt_y[:N], cells, (0.0, 1.0, 0.0, 1.0), OracleBackend(order=2),
+                     motion=OracleMotionReferenceFix(), integrator=2)
+    while run.time < 0.225:
+        run.step()
+    gd, gp, gv = grade(run)
+    print(f"t = {run.time:.4f}: goodness of fit {gd:.4f} {gp:.4f} {gv:.4f}")
+    assert gd < 0.067 and gp < 0.053 and gv < 0.062
