@@ -242,6 +242,27 @@ __global__ void __launch_bounds__(1024) k_reduce_min(const double* __restrict__ 
   }
 }
 
+// RiemannSolver on n pairs of face-frame Primitives (SoA [6][n]: density,
+// pressure, x and y velocity, energy, sound speed), face velocities w (null:
+// zero), fluxes out [4][n] (mass, energy, x and y momentum).
+__global__ void __launch_bounds__(256) k_riemann(const double* __restrict__ l,
+                                                 const double* __restrict__ r,
+                                                 const double* __restrict__ w,
+                                                 double* __restrict__ out, int n, DErr err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = static_cast<size_t>(n);
+  const Prim L{l[i], l[N + i], l[2 * N + i], l[3 * N + i], l[4 * N + i], l[5 * N + i]};
+  const Prim R{r[i], r[N + i], r[2 * N + i], r[3 * N + i], r[4 * N + i], r[5 * N + i]};
+  Cons f{0, 0, 0, 0};
+  const int rc = hllc_prim(L, R, w ? w[i] : 0.0, f);
+  if (rc) report(err, i, rc);
+  out[i] = f.m;
+  out[N + i] = f.E;
+  out[2 * N + i] = f.mx;
+  out[3 * N + i] = f.my;
+}
+
 // ------------------------------------------------------------------ mesh --
 // Per-face static tables.
 __global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceFrame* fframe,

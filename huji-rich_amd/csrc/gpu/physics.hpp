@@ -142,4 +142,49 @@ __host__ __device__ __forceinline__ int hllc_face(const Cell& L, const Cell& R, 
   return 0;
 }
 
+// Hllc::operator() (hllc.cpp:97-154) on two Primitives already in the face
+// frame (x along the normal), with the energy and sound speed they carry,
+// for a face moving at `velocity` along the normal: the same operations as
+// hllc_face after its rotation, the upwind side's flux only.
+__host__ __device__ __forceinline__ int hllc_prim(const Prim& L, const Prim& R, double velocity,
+                                                  Cons& out) {
+  if (is_nan_ref(R.vx)) return ERR_HLLC_NAN;
+  // Velocity -= velocity*normaldir, normaldir = (1,0)
+  const double ul = L.vx - 1.0 * velocity, ur = R.vx - 1.0 * velocity;
+  const double sl = smin(ul - L.c, ur - R.c);
+  const double sr = smax(ul + L.c, ur + R.c);
+  const double ss = (R.p - L.p + L.d * ul * (sl - ul) - R.d * ur * (sr - ur)) /
+                    (L.d * (sl - ul) - R.d * (sr - ur));
+  const int branch = (sl > 0) ? 0 : (sl <= 0 && ss >= 0) ? 1 : (ss < 0 && sr >= 0) ? 2
+                                                            : (sr < 0) ? 3 : 4;
+  if (branch == 4) return ERR_INVALID_WAVE_SPEEDS;
+  const bool left = branch <= 1;
+  const Prim& S = left ? L : R;
+  const double sk = left ? sl : sr;
+  const Prim s{S.d, S.p, left ? ul : ur, S.vy - 0.0 * velocity, S.e, S.c};
+  const double un = s.vx * 1.0 + s.vy * 0.0;
+  const double dun = s.d * un;
+  const double es = ted(s);
+  Cons f;
+  if (branch == 0 || branch == 3) {
+    f.m = s.d * un;
+    f.mx = 1.0 * s.p + s.vx * dun;
+    f.my = 0.0 * s.p + s.vy * dun;
+    f.E = (es + s.p) * un;
+  } else {
+    const double ds = s.d * (sk - s.vx) / (sk - ss);  // starred_state, hllc.cpp:76-92
+    const double usE = es * ds / s.d + ds * (ss - s.vx) * (ss + s.p / s.d / (sk - s.vx));
+    f.m = s.d * un + sk * (ds - s.d);
+    f.mx = (1.0 * s.p + s.vx * dun) + (ds * ss - s.vx * s.d) * sk;
+    f.my = (0.0 * s.p + s.vy * dun) + (ds * s.vy - s.vy * s.d) * sk;
+    f.E = (es + s.p) * un + sk * (usE - es);
+  }
+  f.E = f.E + (f.mx * (1.0 * velocity) + f.my * (0.0 * velocity) + 0.5 * f.m * velocity * velocity);
+  const double vm = velocity * f.m;
+  f.mx = f.mx + 1.0 * vm;
+  f.my = f.my + 0.0 * vm;
+  out = f;
+  return 0;
+}
+
 }  // namespace rg

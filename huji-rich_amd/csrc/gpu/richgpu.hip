@@ -45,6 +45,7 @@ enum KernelId : int {
   KID_ENS,
   KID_MOVING,
   KID_FLUX_PCM,
+  KID_RIEMANN,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -53,7 +54,7 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
                                              "k_adj_slopes",  "k_adj_cell",     "k_ens",
-                                             "k_moving",      "k_flux_pcm"};
+                                             "k_moving",      "k_flux_pcm",     "k_riemann"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -192,6 +193,8 @@ struct rg_ctx {
   int pv_cap = 0;
   D4* keep = nullptr;
   size_t keep_cap = 0;
+  double* riemann_buf = nullptr;  // rg_riemann's states and fluxes
+  size_t riemann_cap = 0;         // doubles
   bool pv_valid = false;
 
   // the rigid box (x0, x1, y0, y1) of rg_set_box, for device geometry
@@ -387,6 +390,15 @@ struct ErrSrc {
   const D4* ext[2];
 };
 
+// hllc.cpp:54-70 adds them in this order: left (density, pressure, x and y
+// velocity, sound speed, energy), right, the interface velocity, then the
+// three wave speeds
+const char* const kWaveEntryNames[16] = {
+    "left density",     "left pressure",     "left x velocity",  "left y velocity",
+    "left sound speed", "left energy",       "right density",    "right pressure",
+    "right x velocity", "right y velocity",  "right sound speed", "right energy",
+    "interface velocity", "left wave speed", "center wave speed", "right wave speed"};
+
 void add_entry(rg_error& e, const char* name, double v) {
   if (e.n_entries >= 16) return;
   rg_error_entry& x = e.entries[e.n_entries++];
@@ -454,15 +466,7 @@ hipError_t fill_entries(rg_ctx* c, rg_error& e, const ErrSrc& src) {
       r = hipMemcpyAsync(v, c->staging, sizeof(v), hipMemcpyDeviceToHost, c->stream);
     if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
     if (r != hipSuccess) return r;
-    static const char* const names[16] = {
-        "left density",     "left pressure",     "left x velocity",  "left y velocity",
-        "left sound speed", "left energy",       "right density",    "right pressure",
-        "right x velocity", "right y velocity",  "right sound speed", "right energy",
-        "interface velocity", "left wave speed", "center wave speed", "right wave speed"};
-    // hllc.cpp:54-70 adds them in this order: left (density, pressure, x and
-    // y velocity, sound speed, energy), right, the interface velocity, then
-    // the three wave speeds
-    for (int k = 0; k < 16; ++k) add_entry(e, names[k], v[k]);
+    for (int k = 0; k < 16; ++k) add_entry(e, kWaveEntryNames[k], v[k]);
   }
   return hipSuccess;
 }
@@ -955,6 +959,7 @@ void rg_destroy(rg_ctx* c) {
   drain_pool(c);
   if (c->pv) (void)hipFree(c->pv);
   if (c->keep) (void)hipFree(c->keep);
+  if (c->riemann_buf) (void)hipFree(c->riemann_buf);
   if (c->dt_hist) (void)hipFree(c->dt_hist);
   if (c->words) (void)hipFree(c->words);
   if (c->host_words) (void)hipHostFree(c->host_words);
@@ -1506,6 +1511,63 @@ int rg_flux(rg_ctx* c, double* fm, double* fE, double* fMx, double* fMy) {
   if (!fm && !fE && !fMx && !fMy) return RG_OK;
   double* const dst[4] = {fm, fE, fMx, fMy};
   return get4(c, dst, c->flux, c->F);
+}
+
+int rg_riemann(rg_ctx* c, int32_t n, const double* left, const double* right,
+               const double* velocity, double* out) {
+  if (!c || n < 0 || (n > 0 && (!left || !right || !out))) return RG_E_ARG;
+  if (n == 0) return RG_OK;
+  const size_t N = static_cast<size_t>(n), need = 17 * N;
+  if (c->riemann_cap < need) {
+    if (c->riemann_buf) (void)hipFree(c->riemann_buf);
+    c->riemann_buf = nullptr;
+    c->riemann_cap = 0;
+    HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&c->riemann_buf), sizeof(double) * need));
+    c->riemann_cap = need;
+  }
+  double* const l = c->riemann_buf;
+  double* const r = l + 6 * N;
+  double* const w = r + 6 * N;
+  double* const f = w + N;
+  HIPCHK(c, hipMemcpyAsync(l, left, sizeof(double) * 6 * N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(r, right, sizeof(double) * 6 * N, hipMemcpyHostToDevice, c->stream));
+  if (velocity)
+    HIPCHK(c, hipMemcpyAsync(w, velocity, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch(c, KID_RIEMANN, [&] {
+    k_riemann<<<blocks_for(n), kBlock, 0, c->stream>>>(l, r, velocity ? w : nullptr, f, n,
+                                                       derr(c, 0, 2));
+  }));
+  HIPCHK(c, hipMemcpyAsync(out, f, sizeof(double) * 4 * N, hipMemcpyDeviceToHost, c->stream));
+  const int rc = collect_error(c);
+  if (rc != RG_OK && c->last.code == RG_ERR_INVALID_WAVE_SPEEDS && c->last.index >= 0 &&
+      c->last.index < n) {
+    // the entries of hllc.cpp:54-70 from the failing pair's inputs
+    const size_t i = static_cast<size_t>(c->last.index);
+    const double v = velocity ? velocity[i] : 0.0;
+    const double* const side[2] = {left, right};
+    double ent[16], u[2], cs[2];
+    for (int k = 0; k < 2; ++k) {
+      const double* s = side[k];
+      u[k] = s[2 * N + i] - 1.0 * v;
+      cs[k] = s[5 * N + i];
+      ent[6 * k + 0] = s[i];
+      ent[6 * k + 1] = s[N + i];
+      ent[6 * k + 2] = u[k];
+      ent[6 * k + 3] = s[3 * N + i] - 0.0 * v;
+      ent[6 * k + 4] = cs[k];
+      ent[6 * k + 5] = s[4 * N + i];
+    }
+    const double sl = smin(u[0] - cs[0], u[1] - cs[1]);
+    const double sr = smax(u[0] + cs[0], u[1] + cs[1]);
+    const double dl = left[i], dr = right[i], pl = left[N + i], pr = right[N + i];
+    ent[12] = v;
+    ent[13] = sl;
+    ent[14] = (pr - pl + dl * u[0] * (sl - u[0]) - dr * u[1] * (sr - u[1])) /
+              (dl * (sl - u[0]) - dr * (sr - u[1]));
+    ent[15] = sr;
+    for (int k = 0; k < 16; ++k) add_entry(c->last, kWaveEntryNames[k], ent[k]);
+  }
+  return rc;
 }
 
 int rg_update_extensives(rg_ctx* c, double dt) {

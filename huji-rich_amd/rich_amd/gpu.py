@@ -76,6 +76,7 @@ def lib():
     L.rg_moving_begin.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp,
                                   P(ctypes.c_double), vp, vp]
     L.rg_moving_end.argtypes = [vp]
+    L.rg_riemann.argtypes = [vp, ctypes.c_int32, vp, vp, vp, vp]
     for n in ("rg_set_cells", "rg_get_cells", "rg_set_extensives", "rg_get_extensives",
               "rg_set_fluxes", "rg_flux"):
         getattr(L, n).argtypes = [vp, vp, vp, vp, vp]
@@ -273,6 +274,23 @@ class RichGPU:
         f = np.empty((4, self.F))
         self._check(self._lib.rg_flux(self._h, *[_p(f[k]) for k in range(4)]))
         return f
+
+    def riemann(self, left, right, velocity=None):
+        """rg_riemann: HLLC on n pairs of face-frame Primitives, left and
+        right (6, n) (density, pressure, x and y velocity, energy, sound
+        speed), face velocities (n,) or None; returns the fluxes (4, n)
+        (mass, energy, x momentum, y momentum)."""
+        L = np.ascontiguousarray(left, dtype=np.float64)
+        R = np.ascontiguousarray(right, dtype=np.float64)
+        if L.ndim != 2 or L.shape[0] != 6 or R.shape != L.shape:
+            raise ValueError("left and right are (6, n)")
+        n = L.shape[1]
+        W = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64)
+        if W is not None and W.shape != (n,):
+            raise ValueError("velocity is (n,)")
+        out = np.empty((4, n))
+        self._check(self._lib.rg_riemann(self._h, n, _p(L), _p(R), _p(W), _p(out)))
+        return out
 
     def update_extensives(self, dt):
         self._check(self._lib.rg_update_extensives(self._h, float(dt)))

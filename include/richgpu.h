@@ -211,6 +211,17 @@ int rg_reconstruct(rg_ctx* ctx, double* slopes, double* left, double* right);
 int rg_flux(rg_ctx* ctx, double* fm, double* fE, double* fMx, double* fMy);
 /* ExtensiveUpdater: extensives -= / += dt*area*flux over faces. */
 int rg_update_extensives(rg_ctx* ctx, double dt);
+/* RiemannSolver::operator()(left, right, velocity) (riemann_solver.hpp:22-23)
+ * -> Hllc (hllc.cpp:97-154), for n pairs at once: Primitives already in the
+ * face frame (x along the normal), left and right [6][n] (density,
+ * pressure, x velocity, y velocity, energy, sound speed, as the caller's
+ * EquationOfState filled them), the face velocity along the normal
+ * velocity [n] (NULL: 0), into the conserved fluxes out [4][n] (mass,
+ * energy, x momentum, y momentum).  Needs no mesh.  A failing pair gives the
+ * reference's message, its index and, for invalid wave speeds, the
+ * entries hllc.cpp:54-70 adds. */
+int rg_riemann(rg_ctx* ctx, int32_t n, const double* left, const double* right,
+               const double* velocity, double* out);
 /* CellUpdater: cells from extensives and CacheData volumes. */
 int rg_update_cells(rg_ctx* ctx);
 
