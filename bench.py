@@ -8,11 +8,12 @@ hdsim::TimeAdvance2Heun pass (two flux stages) over all cells; inputs are
 resident in HBM before the timed region.
 
 Prints ONE JSON line (rank 0).  `value` is Mcell-updates/s (cells x steps /
-s).  `roofline` is one flux stage (k_slopes, k_flux2, k_update and
-k_reduce_min; a Heun step has two): the algorithmic bytes of a stage,
-180 N + 48 F (each datum of the step read or written once), over the
-stage's summed HIP-event kernel time; `traffic` is the HBM bytes the
-counters measured for the same kernels.  `kernels` gives each kernel's
+s).  `roofline` is one flux stage (k_slopes, k_face_update -- the faces
+and the cell update fused per tile of 256 cells -- and k_reduce_min; a Heun
+step has two): the algorithmic bytes of a stage, 180 N + 48 F (each datum
+of the step read or written once), over the stage's summed HIP-event kernel
+time; `traffic` is the HBM bytes the counters measured for the same
+kernels.  `kernels` gives each kernel's
 time and its own model bytes (every array it touches, once), and
 `stage_roofline` the same algorithmic bytes over the timed wall clock.
 `cpu_baseline` times the CPU restatement of the reference step (oracle/,
@@ -32,10 +33,22 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBS = 8000.0
 
 
-STAGE_KERNELS = ("k_slopes", "k_flux2", "k_update", "k_reduce_min")
+STAGE_KERNELS = ("k_slopes", "k_face_update", "k_reduce_min")
+TILE = 256  # cells per tile of k_face_update
 
 
-def kernel_alg_bytes(name, N, F, G, nnz, moving=False):
+def tile_entries(mesh):
+    """Faces k_face_update computes per stage: each face once per tile that
+    holds one of its real sides."""
+    import numpy as np
+    N = mesh.n_cells
+    a, b = np.asarray(mesh.face_first), np.asarray(mesh.face_second)
+    ra, rb = a < N, b < N
+    same = ra & rb & (a // TILE == b // TILE)
+    return int(np.count_nonzero(ra) + np.count_nonzero(rb) - np.count_nonzero(same))
+
+
+def kernel_alg_bytes(name, N, F, G, nnz, moving=False, entries=0):
     """Model bytes of one launch: every array the kernel's own inputs and
     outputs touch, each datum once (FP64 values, int32 indices).  Summed over
     a stage's kernels this counts the intermediates (slopes, fluxes) and the
@@ -53,6 +66,12 @@ def kernel_alg_bytes(name, N, F, G, nnz, moving=False):
         # extensives in 32N, Heun average 32N/2 (one of two stages), flux 32F,
         # area 8F, CSR 4(N+1)+4nnz, volumes 16N; extensives + cells out 64N
         return 32 * N + 16 * N + 32 * F + 8 * F + 4 * (N + 1) + 4 * nnz + 16 * N + 64 * N
+    if name == "k_face_update":
+        # per tile entry: face frame 48 and sides 8; cells 32N, slopes 64N,
+        # CM 16(N+G), ghost ids 8G; extensives in 32N, Heun average 32N/2,
+        # per-slot entry words 2nnz, volumes 16N; extensives + cells out 64N
+        return (56 * entries + 32 * N + 64 * N + 16 * (N + G) + 8 * G
+                + 32 * N + 16 * N + 2 * nnz + 16 * N + 64 * N)
     if name == "k_flux1":
         return 8 * F + 32 * F + 32 * N + 32 * (N + G) + 32 * F
     if name == "k_cfl":
@@ -538,7 +557,7 @@ def main():
         # the CFL minimum after the stage's update)
         stages = 2 * args.profile_steps
         stage_s = sum(prof[k][1] for k in STAGE_KERNELS if k in prof) / stages / 1e3
-        traffic = [measured_traffic(k, args.nx, args.ny) for k in STAGE_KERNELS[:3]]
+        traffic = [measured_traffic(k, args.nx, args.ny) for k in STAGE_KERNELS[:2]]
         achieved = b_alg / stage_s / 1e9
         line["roofline"] = {"bound": "hbm", "kernel": "stage: " + " + ".join(STAGE_KERNELS),
                             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -546,11 +565,14 @@ def main():
                             "traffic": sum(traffic) if None not in traffic else None,
                             "alg_bytes_per_launch": b_alg, "avg_launch_us": stage_s * 1e6,
                             "alg_bytes": "180 N + 48 F per cell-stage"}
+        ent = tile_entries(mesh)
         line["kernels"] = {k: {"launches": n, "avg_us": 1e3 * ms / n,
-                               "model_bytes": kernel_alg_bytes(k, N, F, G, nnz),
-                               "model_GBs": kernel_alg_bytes(k, N, F, G, nnz) / (ms / n / 1e3) / 1e9,
+                               "model_bytes": kernel_alg_bytes(k, N, F, G, nnz, entries=ent),
+                               "model_GBs": kernel_alg_bytes(k, N, F, G, nnz, entries=ent)
+                               / (ms / n / 1e3) / 1e9,
                                "traffic": measured_traffic(k, args.nx, args.ny)}
                            for k, (n, ms) in sorted(prof.items())}
+        line["tile_entries"] = {"entries": ent, "per_face": ent / F, "cells_per_tile": TILE}
     stage_gbs = b_alg * 2 * args.steps / elapsed / 1e9
     line["stage_roofline"] = {"bytes_per_stage": b_alg, "achieved": stage_gbs,
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}

@@ -116,6 +116,29 @@ __device__ __forceinline__ int ell_base(const DMesh& m, int i) {
   return m.wofs[i >> 6] + m.lanepos[i];
 }
 
+// Tiles of the fused face-and-update kernel (k_face_update).  Tile t holds
+// the cells kTile*t .. kTile*t+kTile-1 and, as its entries tofs[t] ..
+// tofs[t+1]-1, every face with a real side among them, once (a face between
+// two tiles is an entry of both).  Per entry the face's frame and sides are
+// stored in entry order, so the face phase streams them; the face id is read
+// only for face velocities and errors.  Per ELL slot, eloc holds the entry of
+// the slot's face within the cell's tile, with TILE_SECOND where the cell is
+// the face's second side.
+constexpr int kTile = 256;
+constexpr unsigned TILE_SECOND = 0x8000u;
+constexpr unsigned TILE_ENTRY = 0x7FFFu;
+constexpr int kTileMaxEntries = 2040;  // fluxes and block_min within 64 KiB of LDS
+struct FaceSides {
+  int a, b;
+};
+struct DTiles {
+  const int* tofs;            // [tiles + 1]
+  const int* tface;           // face id of each entry
+  const FaceSides* tside;     // (first, second) of each entry
+  const FaceFrame* tframe;    // frame of each entry
+  const unsigned short* eloc; // per ELL slot
+};
+
 struct DParams {
   double gamma, cfl, delta_v, theta, delta_p;
   int slf;
@@ -932,11 +955,15 @@ struct UpdArgs {
 // UPD_CFL is set (else +inf).  kEarly issues the loads that do not depend on
 // the face sum (Heun operand, volume, width, error word) before it, so that
 // they do not wait behind the stores (more registers: the one-cell-per-thread
-// kernel only).
-template <bool kEarly = false>
+// kernels only).  With kTileFlux the face terms dt*area*F come from the
+// tile's LDS array `tflux` through the per-slot words `eloc` (entry within
+// the tile | TILE_SECOND) instead of the face array a.flux.
+template <bool kEarly = false, bool kTileFlux = false>
 __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
                                               const DParams& prm, const DErr& err, int i,
-                                              double dt) {
+                                              double dt,
+                                              const unsigned short* __restrict__ eloc = nullptr,
+                                              const D4* tflux = nullptr) {
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   // After a device-detected error the updates stop: the state stays as the
   // failing call left it (the reference throws there), so that the error's
@@ -969,16 +996,21 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
         unsigned inf[CH_UPD];
         D4 fl[CH_UPD];
         double ar[CH_UPD];
-        const bool delta = (a.flags & UPD_DELTA) != 0;
+        const bool delta = kTileFlux || (a.flags & UPD_DELTA) != 0;
 #pragma unroll
         for (int u = 0; u < CH_UPD; ++u)
           if (s0 + u < deg) {
-            inf[u] = m.einfo[eb + 64 * (s0 + u)];
+            if (kTileFlux) {
+              const unsigned w = eloc[eb + 64 * (s0 + u)];
+              inf[u] = (w & TILE_ENTRY) | ((w & TILE_SECOND) ? INFO_SECOND : 0u);
+            } else {
+              inf[u] = m.einfo[eb + 64 * (s0 + u)];
+            }
             ar[u] = delta ? 0.0 : m.earea[eb + 64 * (s0 + u)];
           }
 #pragma unroll
         for (int u = 0; u < CH_UPD; ++u)
-          if (s0 + u < deg) fl[u] = a.flux[inf[u] & INFO_FACE];
+          if (s0 + u < deg) fl[u] = kTileFlux ? tflux[inf[u] & INFO_FACE] : a.flux[inf[u] & INFO_FACE];
 #pragma unroll
         for (int u = 0; u < CH_UPD; ++u) {
           if (s0 + u >= deg) break;
@@ -1052,6 +1084,65 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   if (i < m.N) cfl_v = update_cell<true>(m, a, prm, err, i, dt);
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
+}
+
+// One stage of the Heun step loop in one launch per tile of kTile cells
+// (second order, dt*area*F): ModularFluxCalculator over LinearGaussImproved
+// on every face of the tile (flux2_face's operations, in the same order) into
+// LDS, then k_update's cell update of the tile's cells from those fluxes.
+// The face kernel's output never leaves the chip.  A face on a tile boundary
+// is computed by both tiles, with equal results.  The stage must not write
+// what its faces read: the update's outputs (a.out, a.cells) are other
+// buffers than the cells q and slopes rec.
+__global__ void __launch_bounds__(256) k_face_update(DMesh m, DTiles tl, const D4* __restrict__ q,
+                                                     const Rec* __restrict__ rec, UpdArgs a,
+                                                     DParams prm, DErr errf, DErr erru) {
+  extern __shared__ D4 tflux[];
+  const int t = walk_block(prm);
+  const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
+  const int e0 = tl.tofs[t], ne = tl.tofs[t + 1] - e0;
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+    const int e = e0 + k;
+    const FaceFrame fr = tl.tframe[e];
+    const FaceSides sd = tl.tside[e];
+    Cell lf, rt;
+    face_states(m, q, rec, sd.a, sd.b, fr, lf, rt);
+    const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
+    // modular_flux_calculator.cpp:56-67
+    double wx = 0.0, wy = 0.0;
+    if (m.wx) {
+      const int f = tl.tface[e];
+      wx = m.wx[f];
+      wy = m.wy[f];
+    }
+    const double speed = (fr.nx * wx + fr.ny * wy) / an;
+    Cons c;
+    const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
+    D4 o{0, 0, 0, 0};
+    if (rc) {
+      report(errf, tl.tface[e], rc);
+    } else {
+      const double sc = dt * ap;  // the face area |v1 - v2| (physical_geometry.cpp:8-11)
+      o = D4{c.m * sc, c.E * sc, c.mx * sc, c.my * sc};
+    }
+    tflux[k] = o;
+  }
+  __syncthreads();
+  const int i = t * kTile + threadIdx.x;
+  if (i == 0) {
+    if (a.dt_hist) a.dt_hist[a.step] = dt;
+    if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
+  }
+  double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  if (i < m.N) cfl_v = update_cell<true, true>(m, a, prm, erru, i, dt, tl.eloc, tflux);
+  if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
+}
+
+// The entries' frames in entry order (set-up of the tiles)
+__global__ void __launch_bounds__(256) k_tile_frames(DMesh m, const int* __restrict__ tface,
+                                                     FaceFrame* __restrict__ tframe, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) tframe[e] = m.fframe[tface[e]];
 }
 
 // init_extensives (hdsim2d.cpp:41-66)

@@ -46,6 +46,7 @@ enum KernelId : int {
   KID_MOVING,
   KID_FLUX_PCM,
   KID_RIEMANN,
+  KID_FACE_UPDATE,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -54,7 +55,8 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_pack",        "k_reduce_min", "k_slopes_mixed",
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
                                              "k_adj_slopes",  "k_adj_cell",     "k_ens",
-                                             "k_moving",      "k_flux_pcm",     "k_riemann"};
+                                             "k_moving",      "k_flux_pcm",     "k_riemann",
+                                             "k_face_update"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -193,6 +195,14 @@ struct rg_ctx {
   int pv_cap = 0;
   D4* keep = nullptr;
   size_t keep_cap = 0;
+  // tiles of the fused face-and-update kernel, built at the first Heun step
+  // loop of a mesh (tiles_ok: they fit the kernel's LDS array), and the
+  // second state buffers that loop alternates with
+  DTiles tiles{};
+  int tiles_max = 0;
+  bool tiles_built = false, tiles_ok = false;
+  std::vector<void*> tile_bufs;
+  D4 *alt_prim = nullptr, *alt_ext = nullptr;
   double* riemann_buf = nullptr;  // rg_riemann's states and fluxes
   size_t riemann_cap = 0;         // doubles
   bool pv_valid = false;
@@ -289,6 +299,12 @@ void free_mesh(rg_ctx* c, bool pool = false) {
   c->primM = c->mid_primM = c->fluxM = nullptr;
   for (void* p : c->adj_bufs) (void)hipFree(p);
   c->adj_bufs.clear();
+  for (void* p : c->tile_bufs) (void)hipFree(p);
+  c->tile_bufs.clear();
+  c->tiles = DTiles{};
+  c->tiles_max = 0;
+  c->tiles_built = c->tiles_ok = false;
+  c->alt_prim = c->alt_ext = nullptr;
   c->am = AdjMesh{};
   c->adj_src = rg_ctx::AdjSource{};
   c->adj_steps = c->adj_every = c->adj_nck = c->adj_cap_every = c->adj_cap_nck = 0;
@@ -644,6 +660,156 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   return run_reduce_min(c, nxt);
 }
 
+// The tiles of k_face_update (second-order FP64 contexts, built at the
+// first Heun step loop of a mesh from the host copy of its incidences) and
+// the second state buffers.  A face with both sides in one tile becomes an
+// entry at its first side; every other incidence of a real cell makes its
+// own entry.  Entries follow the tile's cells and their faces in ascending
+// order.  A mesh whose largest tile would hold more than kTileMaxEntries
+// faces keeps the three-kernel stage (tiles_ok false).
+hipError_t build_tiles(rg_ctx* c) {
+  c->tiles_built = true;
+  c->tiles_ok = false;
+  const auto& s = c->adj_src;
+  const int N = c->N, nt = (N + kTile - 1) / kTile;
+  if (s.inc.size() != static_cast<size_t>(c->nnz) || s.csr_off.size() != static_cast<size_t>(N) + 1)
+    return hipSuccess;
+  std::vector<int> tofs(static_cast<size_t>(nt) + 1, 0);
+  std::vector<int> loc(static_cast<size_t>(std::max(c->nnz, 1)), -1);
+  auto inside = [&](int j, int t) { return j < N && j / kTile == t; };
+  // owners: their entry within the tile
+  rmp::parallel_for(nt, [&](int64_t tt) {
+    const int t = static_cast<int>(tt), i1 = std::min(N, (t + 1) * kTile);
+    int n = 0;
+    for (int i = t * kTile; i < i1; ++i)
+      for (int k = s.csr_off[i]; k < s.csr_off[i + 1]; ++k)
+        if (!inside(s.inc[k].nbr, t) || !(s.inc[k].info & INFO_SECOND)) loc[k] = n++;
+    tofs[static_cast<size_t>(t) + 1] = n;
+  }, 64);
+  int mx = 0;
+  for (int t = 0; t < nt; ++t) mx = std::max(mx, tofs[static_cast<size_t>(t) + 1]);
+  if (mx > kTileMaxEntries) return hipSuccess;
+  for (int t = 0; t < nt; ++t) tofs[static_cast<size_t>(t) + 1] += tofs[t];
+  const int total = tofs[nt];
+  std::vector<int> tface(static_cast<size_t>(std::max(total, 1)), 0);
+  std::vector<FaceSides> tside(static_cast<size_t>(std::max(total, 1)), FaceSides{0, 0});
+  std::vector<unsigned short> eloc(static_cast<size_t>(std::max(c->dm_ne, 1)), 0);
+  bool bad = false;
+  rmp::parallel_for(nt, [&](int64_t tt) {
+    const int t = static_cast<int>(tt), i1 = std::min(N, (t + 1) * kTile);
+    for (int i = t * kTile; i < i1; ++i) {
+      const size_t base = static_cast<size_t>(s.wofs[i / 64] + s.lanepos[static_cast<size_t>(i)]);
+      for (int k = s.csr_off[i]; k < s.csr_off[i + 1]; ++k) {
+        const unsigned info = s.inc[k].info;
+        const int f = static_cast<int>(info & INFO_FACE), j = s.inc[k].nbr;
+        int l = loc[k];
+        if (l < 0) {
+          // second side of a face inside the tile: the first side's entry
+          for (int kk = s.csr_off[j]; kk < s.csr_off[j + 1]; ++kk)
+            if (static_cast<int>(s.inc[kk].info & INFO_FACE) == f) l = loc[kk];
+          if (l < 0) {
+            bad = true;  // a face listed by one of its sides only
+            return;
+          }
+        } else {
+          const size_t e = static_cast<size_t>(tofs[t] + l);
+          tface[e] = f;
+          tside[e] = FaceSides{s.ff[f], s.fs[f]};
+        }
+        eloc[base + 64 * static_cast<size_t>(k - s.csr_off[i])] = static_cast<unsigned short>(
+            static_cast<unsigned>(l) | ((info & INFO_SECOND) ? TILE_SECOND : 0u));
+      }
+    }
+  }, 64);
+  if (bad) return hipSuccess;
+  auto get = [&](auto** p, size_t n) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, sizeof(**p) * std::max<size_t>(n, 1));
+    if (e == hipSuccess) c->tile_bufs.push_back(q);
+    *p = static_cast<std::remove_reference_t<decltype(*p)>>(q);
+    return e;
+  };
+  int *tofs_d, *tface_d;
+  FaceSides* tside_d;
+  FaceFrame* tframe_d;
+  unsigned short* eloc_d;
+  hipError_t e = get(&tofs_d, tofs.size());
+  if (e == hipSuccess) e = get(&tface_d, tface.size());
+  if (e == hipSuccess) e = get(&tside_d, tside.size());
+  if (e == hipSuccess) e = get(&tframe_d, tface.size());
+  if (e == hipSuccess) e = get(&eloc_d, eloc.size());
+  if (e == hipSuccess) e = get(&c->alt_prim, static_cast<size_t>(N));
+  if (e == hipSuccess) e = get(&c->alt_ext, static_cast<size_t>(N));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(tofs_d, tofs.data(), sizeof(int) * tofs.size(), hipMemcpyHostToDevice,
+                     c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tface_d, tface.data(), sizeof(int) * tface.size(), hipMemcpyHostToDevice,
+                       c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tside_d, tside.data(), sizeof(FaceSides) * tside.size(),
+                       hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(eloc_d, eloc.data(), sizeof(unsigned short) * eloc.size(),
+                       hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return e;
+  e = launch(c, KID_GEO, [&] {
+    k_tile_frames<<<blocks_for(std::max(total, 1)), kBlock, 0, c->stream>>>(c->dm, tface_d,
+                                                                           tframe_d, total);
+  });
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the host vectors are uploaded
+  if (e != hipSuccess) return e;
+  c->tiles = DTiles{tofs_d, tface_d, tside_d, tframe_d, eloc_d};
+  c->tiles_max = mx;
+  c->tiles_ok = true;
+  return hipSuccess;
+}
+
+// One stage of the fused Heun step: the slope kernel, then k_face_update
+// walking its tiles in reverse (so that it starts on the slopes written
+// last).  dt from slot `cur` as in heun_step.
+hipError_t fused_stage(rg_ctx* c, const D4* cells, Rec* r, const UpdArgs& a, int s,
+                       int phase_base) {
+  hipError_t e = run_slopes(c, cells, s, phase_base, r, 0);
+  if (e != hipSuccess) return e;
+  const size_t lds = sizeof(D4) * static_cast<size_t>(c->tiles_max);
+  return launch(c, KID_FACE_UPDATE, [&] {
+    k_face_update<<<blocks_for(c->N), kTile, lds, c->stream>>>(
+        c->dm, c->tiles, cells, r, a, dparams_walk(c, 1), derr(c, s, phase_base + 1),
+        derr(c, s, phase_base + 2));
+  });
+}
+
+// heun_step with each stage's faces and update fused per tile
+// (k_face_update).  The end state must not be the start state: the second
+// stage's faces read the mid state only, but a failed step has to leave the
+// start state as it was.
+hipError_t heun_step_fused(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs& h) {
+  const int cur = s & 1, nxt = (s + 1) & 1;
+  UpdArgs a{};
+  a.in = h.X;
+  a.out = h.Xm;
+  a.cells = h.Pm;
+  a.dt_slot = c->words + cur;
+  a.dt_hist = dt_hist;
+  a.step = hist;
+  a.flags = UPD_FACES | UPD_DELTA | UPD_ZF_BEFORE | UPD_CELLS;
+  hipError_t e = fused_stage(c, h.P, h.r1, a, s, 1);
+  if (e != hipSuccess) return e;
+  UpdArgs b{};
+  b.in = h.Xm;
+  b.out = h.Xo;
+  b.avg = h.X;
+  b.cells = h.Po;
+  b.dt_slot = c->words + cur;
+  b.partials = c->partials;
+  b.step = hist;
+  b.flags = UPD_FACES | UPD_DELTA | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
+  e = fused_stage(c, h.Pm, h.r2, b, s, 4);
+  if (e != hipSuccess) return e;
+  return run_reduce_min(c, nxt);
+}
+
 Ens ens_args(rg_ctx* c) {
   return Ens{c->ens_K,   c->ens_inst, c->ens_coff,     c->ens_dtA,  c->ens_dtB,
              c->ens_inv, c->ens_hist, c->ens_step,     c->ens_hist_cap};
@@ -851,8 +1017,19 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
 
 // The FP64 step loop of rg_advance on a static mesh: the dt slots, block
 // minima and dt history are the context's words, partials and dt_hist
-// (record: write each step's dt to dt_hist).
-hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record) {
+// (record: write each step's dt to dt_hist).  Second-order Heun steps run
+// fused (heun_step_fused) where the mesh has tiles -- built here at the
+// first call of a mesh when may_build -- and the state then alternates
+// between the context's two state buffers: step s reads one pair and writes
+// the other, and c->prim / c->ext name the newest.  *fused tells which.
+hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record, bool may_build,
+                        bool* fused) {
+  const bool can_fuse = integrator == 2 && c->prm.order == 2;
+  if (can_fuse && may_build && !c->tiles_built) {
+    const hipError_t eb = build_tiles(c);
+    if (eb != hipSuccess) return eb;
+  }
+  *fused = can_fuse && c->tiles_ok;
   // first dt: SimpleCFL on the current cells into slot 0
   hipError_t e = run_cfl(c, c->prim, 0);
   for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
@@ -873,6 +1050,14 @@ hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record) {
       a.flags = UPD_FACES | UPD_ZF_AFTER | UPD_CELLS | UPD_CFL;
       e = run_update(c, a, s, 3);
       if (e == hipSuccess) e = run_reduce_min(c, nxt);
+    } else if (*fused) {
+      const HeunBufs h{c->prim, c->ext, c->mid_prim, c->mid_ext,
+                       c->alt_prim, c->alt_ext, c->rec, c->rec};
+      e = heun_step_fused(c, s, record ? c->dt_hist : nullptr, s, h);
+      if (e == hipSuccess) {
+        std::swap(c->prim, c->alt_prim);
+        std::swap(c->ext, c->alt_ext);
+      }
     } else {
       e = heun_step(c, s, record ? c->dt_hist : nullptr, s, ctx_bufs(c));
     }
@@ -885,6 +1070,29 @@ ErrSrc fp64_err_src(const rg_ctx* c, int integrator) {
   if (integrator == 1) return ErrSrc{{c->prim, nullptr}, {c->ext, nullptr}};
   const HeunBufs h = ctx_bufs(const_cast<rg_ctx*>(c));
   return ErrSrc{{h.P, h.Pm}, {h.Xm, h.Xo}};
+}
+
+// collect_error after advance_fp64.  A fused loop that failed at step s
+// goes back to the buffers step s started from (the steps after a failure
+// write no state), and the error's entries come from step s's buffers.
+int collect_fp64(rg_ctx* c, int integrator, int nsteps, bool fused) {
+  if (!fused) {
+    const ErrSrc src = fp64_err_src(c, integrator);
+    return collect_error(c, &src);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->host_words, c->words, 3 * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const unsigned long long w = c->host_words[2];
+  if (w != ~0ull) {
+    const int step = static_cast<int>(w >> 39);
+    if (step >= 0 && step < nsteps && ((nsteps - step) & 1)) {
+      std::swap(c->prim, c->alt_prim);
+      std::swap(c->ext, c->alt_ext);
+    }
+  }
+  const ErrSrc src{{c->prim, c->mid_prim}, {c->mid_ext, c->alt_ext}};
+  return collect_error(c, &src);
 }
 
 // For the duration of a rg_advance_noalloc call: the context's dt slots,
@@ -1639,12 +1847,12 @@ int rg_advance(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
   }
   if (c->prm.precision == RG_MIXED && integrator == 2 && c->prm.order == 2)
     return advance_mixed(c, nsteps, dt_out);
-  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr));
+  bool fused = false;
+  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr, true, &fused));
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
-  const ErrSrc src = fp64_err_src(c, integrator);
-  return collect_error(c, &src);
+  return collect_fp64(c, integrator, nsteps, fused);
 }
 
 int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt_out) {
@@ -1669,12 +1877,14 @@ int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt
   // the scratch keeps the last call's contents: start with clear dt slots
   // and error word (a fill kernel, no memset)
   HIPCHK(c, fill_words(c, 0, 3, ~0ull));
-  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr));
+  // the fused stages only where an earlier rg_advance built the tiles: this
+  // call allocates nothing
+  bool fused = false;
+  HIPCHK(c, advance_fp64(c, integrator, nsteps, dt_out != nullptr, false, &fused));
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
-  const ErrSrc src = fp64_err_src(c, integrator);
-  return collect_error(c, &src);
+  return collect_fp64(c, integrator, nsteps, fused);
 }
 
 int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out) {
