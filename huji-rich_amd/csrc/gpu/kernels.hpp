@@ -958,12 +958,46 @@ struct UpdArgs {
 // kernels only).  With kTileFlux the face terms dt*area*F come from the
 // tile's LDS array `tflux` through the per-slot words `eloc` (entry within
 // the tile | TILE_SECOND) instead of the face array a.flux.
+// The loads of a cell's update that do not depend on the face sum (kEarly):
+// extensives, Heun operand, volume, width, error word and, for the tile
+// form, the first CH_UPD slot words.
+struct UpdLoads {
+  D4 x, o;
+  double vol, wid;
+  int deg, eb;
+  bool frozen;
+  unsigned w[CH_UPD];
+};
+
+template <bool kTileFlux = false>
+__device__ __forceinline__ UpdLoads update_loads(const DMesh& m, const UpdArgs& a, const DErr& err,
+                                                 int i, const unsigned short* __restrict__ eloc) {
+  UpdLoads L;
+  L.o = D4{0, 0, 0, 0};
+  L.vol = L.wid = 0;
+  if (a.flags & UPD_AVERAGE) L.o = a.avg[i];
+  if (a.flags & UPD_CELLS) {
+    L.vol = m.vol[i];
+    if (a.flags & UPD_CFL) L.wid = m.width[i];
+  }
+  L.frozen = *err.word != ~0ull;
+  L.x = a.in[i];
+  L.deg = m.off[i + 1] - m.off[i];
+  L.eb = ell_base(m, i);
+  if (kTileFlux) {
+#pragma unroll
+    for (int u = 0; u < CH_UPD; ++u) L.w[u] = u < L.deg ? eloc[L.eb + 64 * u] : 0u;
+  }
+  return L;
+}
+
 template <bool kEarly = false, bool kTileFlux = false>
 __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
                                               const DParams& prm, const DErr& err, int i,
                                               double dt,
                                               const unsigned short* __restrict__ eloc = nullptr,
-                                              const D4* tflux = nullptr) {
+                                              const D4* tflux = nullptr,
+                                              const UpdLoads* pre = nullptr) {
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   // After a device-detected error the updates stop: the state stays as the
   // failing call left it (the reference throws there), so that the error's
@@ -971,18 +1005,18 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
   bool frozen = false;
   D4 o{0, 0, 0, 0};
   double vol = 0, wid = 0;
+  UpdLoads L{};
   if (kEarly) {
-    if (a.flags & UPD_AVERAGE) o = a.avg[i];
-    if (a.flags & UPD_CELLS) {
-      vol = m.vol[i];
-      if (a.flags & UPD_CFL) wid = m.width[i];
-    }
-    frozen = *err.word != ~0ull;
+    L = pre ? *pre : update_loads<kTileFlux>(m, a, err, i, eloc);
+    o = L.o;
+    vol = L.vol;
+    wid = L.wid;
+    frozen = L.frozen;
   } else if (*err.word != ~0ull) {
     return cfl_v;
   }
   {
-    const D4 x = a.in[i];
+    const D4 x = kEarly ? L.x : a.in[i];
     double em = x.a, eE = x.b, ex = x.c, ey = x.d;
     if (a.flags & UPD_ZF_BEFORE) {
       em += dt * 0.0;
@@ -991,7 +1025,8 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
       eE += dt * 0.0;
     }
     if (a.flags & UPD_FACES) {
-      const int deg = m.off[i + 1] - m.off[i], eb = ell_base(m, i);
+      const int deg = kEarly ? L.deg : m.off[i + 1] - m.off[i];
+      const int eb = kEarly ? L.eb : ell_base(m, i);
       for (int s0 = 0; s0 < deg; s0 += CH_UPD) {
         unsigned inf[CH_UPD];
         D4 fl[CH_UPD];
@@ -1001,7 +1036,7 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
         for (int u = 0; u < CH_UPD; ++u)
           if (s0 + u < deg) {
             if (kTileFlux) {
-              const unsigned w = eloc[eb + 64 * (s0 + u)];
+              const unsigned w = (kEarly && s0 == 0) ? L.w[u] : eloc[eb + 64 * (s0 + u)];
               inf[u] = (w & TILE_ENTRY) | ((w & TILE_SECOND) ? INFO_SECOND : 0u);
             } else {
               inf[u] = m.einfo[eb + 64 * (s0 + u)];
@@ -1101,10 +1136,21 @@ __global__ void __launch_bounds__(256) k_face_update(DMesh m, DTiles tl, const D
   const int t = walk_block(prm);
   const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
   const int e0 = tl.tofs[t], ne = tl.tofs[t + 1] - e0;
+  // each thread's next entry is loaded while the current one is solved
+  FaceFrame frn{0, 0, 0, 0, 0, 0};
+  FaceSides sdn{0, 0};
+  if (static_cast<int>(threadIdx.x) < ne) {
+    frn = tl.tframe[e0 + threadIdx.x];
+    sdn = tl.tside[e0 + threadIdx.x];
+  }
   for (int k = threadIdx.x; k < ne; k += blockDim.x) {
     const int e = e0 + k;
-    const FaceFrame fr = tl.tframe[e];
-    const FaceSides sd = tl.tside[e];
+    const FaceFrame fr = frn;
+    const FaceSides sd = sdn;
+    if (k + static_cast<int>(blockDim.x) < ne) {
+      frn = tl.tframe[e + blockDim.x];
+      sdn = tl.tside[e + blockDim.x];
+    }
     Cell lf, rt;
     face_states(m, q, rec, sd.a, sd.b, fr, lf, rt);
     const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
@@ -1127,14 +1173,17 @@ __global__ void __launch_bounds__(256) k_face_update(DMesh m, DTiles tl, const D
     }
     tflux[k] = o;
   }
-  __syncthreads();
+  // the update's loads that need no flux are in flight across the barrier
   const int i = t * kTile + threadIdx.x;
+  UpdLoads L{};
+  if (i < m.N) L = update_loads<true>(m, a, erru, i, tl.eloc);
+  __syncthreads();
   if (i == 0) {
     if (a.dt_hist) a.dt_hist[a.step] = dt;
     if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
   }
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
-  if (i < m.N) cfl_v = update_cell<true, true>(m, a, prm, erru, i, dt, tl.eloc, tflux);
+  if (i < m.N) cfl_v = update_cell<true, true>(m, a, prm, erru, i, dt, tl.eloc, tflux, &L);
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
 }
 
