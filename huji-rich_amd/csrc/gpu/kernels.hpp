@@ -131,11 +131,13 @@ constexpr int kTileMaxEntries = 2040;  // fluxes and block_min within 64 KiB of 
 struct FaceSides {
   int a, b;
 };
+struct FaceM;  // the mixed-precision face record
 struct DTiles {
   const int* tofs;            // [tiles + 1]
   const int* tface;           // face id of each entry
   const FaceSides* tside;     // (first, second) of each entry
-  const FaceFrame* tframe;    // frame of each entry
+  const FaceFrame* tframe;    // frame of each entry (FP64)
+  const FaceM* tfm;           // FP32 face record of each entry (mixed precision)
   const unsigned short* eloc; // per ELL slot
 };
 
@@ -961,12 +963,15 @@ struct UpdArgs {
 // The loads of a cell's update that do not depend on the face sum (kEarly):
 // extensives, Heun operand, volume, width, error word and, for the tile
 // form, the first CH_UPD slot words.
+// Slots per chunk of the tile form's update (4 measured slower: 200.8 ->
+// 206.5 us on a 2M-cell mesh, at the same register count).
+constexpr int CH_TILE = CH_UPD;
 struct UpdLoads {
   D4 x, o;
   double vol, wid;
   int deg, eb;
   bool frozen;
-  unsigned w[CH_UPD];
+  unsigned w[CH_TILE];
 };
 
 template <bool kTileFlux = false>
@@ -986,7 +991,7 @@ __device__ __forceinline__ UpdLoads update_loads(const DMesh& m, const UpdArgs& 
   L.eb = ell_base(m, i);
   if (kTileFlux) {
 #pragma unroll
-    for (int u = 0; u < CH_UPD; ++u) L.w[u] = u < L.deg ? eloc[L.eb + 64 * u] : 0u;
+    for (int u = 0; u < CH_TILE; ++u) L.w[u] = u < L.deg ? eloc[L.eb + 64 * u] : 0u;
   }
   return L;
 }
@@ -1027,6 +1032,7 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
     if (a.flags & UPD_FACES) {
       const int deg = kEarly ? L.deg : m.off[i + 1] - m.off[i];
       const int eb = kEarly ? L.eb : ell_base(m, i);
+      constexpr int CH_UPD = kTileFlux ? CH_TILE : ::rg::CH_UPD;
       for (int s0 = 0; s0 < deg; s0 += CH_UPD) {
         unsigned inf[CH_UPD];
         D4 fl[CH_UPD];
@@ -1185,13 +1191,6 @@ __global__ void __launch_bounds__(256) k_face_update(DMesh m, DTiles tl, const D
   double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
   if (i < m.N) cfl_v = update_cell<true, true>(m, a, prm, erru, i, dt, tl.eloc, tflux, &L);
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
-}
-
-// The entries' frames in entry order (set-up of the tiles)
-__global__ void __launch_bounds__(256) k_tile_frames(DMesh m, const int* __restrict__ tface,
-                                                     FaceFrame* __restrict__ tframe, int n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) tframe[e] = m.fframe[tface[e]];
 }
 
 // init_extensives (hdsim2d.cpp:41-66)

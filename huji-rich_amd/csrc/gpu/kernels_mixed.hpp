@@ -413,6 +413,11 @@ __device__ __forceinline__ int hllc_face_m(const CellF& L, const CellF& R, float
   return 0;
 }
 
+// the face's speed along its unit normal
+__device__ __forceinline__ float face_speed_m(const FaceM& fr, float wx, float wy) {
+  return fr.nx * wx + fr.ny * wy;
+}
+
 __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4* __restrict__ q,
                                                     const RecM* __restrict__ rec,
                                                     F4* __restrict__ flux, DParams prm, DErr err) {
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4
   face_states_m(m, q, rec, m.ff[f], m.fs[f], fr, lt, rt);
   const float wx = m.wx ? static_cast<float>(m.wx[f]) : 0.f;
   const float wy = m.wy ? static_cast<float>(m.wy[f]) : 0.f;
-  const float speed = fr.nx * wx + fr.ny * wy;
+  const float speed = face_speed_m(fr, wx, wy);
   F4 o{0, 0, 0, 0};
   const int rc = hllc_face_m(lt, rt, static_cast<float>(prm.gamma), fr.nx, fr.ny, -fr.ny, fr.nx,
                              fr.area, speed, o);
@@ -516,6 +521,135 @@ __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArg
     if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, m.cgeo[i].width, Cell{d, p, vx, vy});
   }
   if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
+}
+
+// k_face_update in mixed precision: k_flux_mixed on every face the tile's
+// cells read (a periodic boundary face through its lower-numbered copy, as
+// k_update_mixed reads it) into LDS, then k_update_mixed's update of the
+// tile's cells from those fluxes.  The update's outputs are other buffers
+// than the cells and slopes the faces read.
+__global__ void __launch_bounds__(256) k_face_update_mixed(DMesh m, MTables t, DTiles tl,
+                                                           const F4* __restrict__ q,
+                                                           const RecM* __restrict__ rec,
+                                                           UpdArgsM a, DParams prm, DErr errf,
+                                                           DErr erru) {
+  extern __shared__ F4 tfluxm[];
+  // slots per chunk of the update: the fluxes come from LDS, so a short
+  // chunk hides their latency and keeps the registers down
+  constexpr int CHT = 4;
+  const int tile = walk_block(prm);
+  const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
+  const int e0 = tl.tofs[tile], ne = tl.tofs[tile + 1] - e0;
+  const float g = static_cast<float>(prm.gamma);
+  // each thread's next entry is loaded while the current one is solved
+  FaceM frn{0, 0, 0, 0, 0, 0, 0, 0};
+  FaceSides sdn{0, 0};
+  if (static_cast<int>(threadIdx.x) < ne) {
+    frn = tl.tfm[e0 + threadIdx.x];
+    sdn = tl.tside[e0 + threadIdx.x];
+  }
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+    const int e = e0 + k;
+    const FaceM fr = frn;
+    const FaceSides sd = sdn;
+    if (k + static_cast<int>(blockDim.x) < ne) {
+      frn = tl.tfm[e + blockDim.x];
+      sdn = tl.tside[e + blockDim.x];
+    }
+    CellF lt, rt;
+    face_states_m(m, q, rec, sd.a, sd.b, fr, lt, rt);
+    float wx = 0.f, wy = 0.f;
+    if (m.wx) {
+      const int f = tl.tface[e];
+      wx = static_cast<float>(m.wx[f]);
+      wy = static_cast<float>(m.wy[f]);
+    }
+    const float speed = face_speed_m(fr, wx, wy);
+    F4 o{0, 0, 0, 0};
+    const int rc = hllc_face_m(lt, rt, g, fr.nx, fr.ny, -fr.ny, fr.nx, fr.area, speed, o);
+    if (rc) {
+      report(errf, tl.tface[e], rc);
+      o = F4{0, 0, 0, 0};
+    }
+    tfluxm[k] = o;
+  }
+  const int i = tile * kTile + threadIdx.x;
+  D4 x{0, 0, 0, 0}, av{0, 0, 0, 0};
+  double vol = 0, wid = 0;
+  int deg = 0, eb = 0;
+  unsigned w0[CHT];  // the first chunk's slot words
+  if (i < m.N) {
+    x = a.in[i];
+    if (a.flags & UPD_AVERAGE) av = a.avg[i];
+    vol = m.vol[i];
+    if (a.flags & UPD_CFL) wid = t.cgeo[i].width;
+    deg = m.off[i + 1] - m.off[i];
+    eb = ell_base(t, i);
+#pragma unroll
+    for (int u = 0; u < CHT; ++u) w0[u] = u < deg ? tl.eloc[eb + 64 * u] : 0u;
+  }
+  __syncthreads();
+  if (i == 0) {
+    if (a.dt_hist) a.dt_hist[a.step] = dt;
+    if (a.reset_slot) *a.reset_slot = DBL_INF_BITS;
+  }
+  double cfl_v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
+  if (i < m.N) {
+    KSum sm{x.a, 0}, sE{x.b, 0}, sx{x.c, 0}, sy{x.d, 0};
+    for (int s0 = 0; s0 < deg; s0 += CHT) {
+      unsigned inf[CHT];
+      F4 fl[CHT];
+#pragma unroll
+      for (int u = 0; u < CHT; ++u)
+        if (s0 + u < deg) inf[u] = s0 == 0 ? w0[u] : tl.eloc[eb + 64 * (s0 + u)];
+#pragma unroll
+      for (int u = 0; u < CHT; ++u)
+        if (s0 + u < deg) fl[u] = tfluxm[inf[u] & TILE_ENTRY];
+#pragma unroll
+      for (int u = 0; u < CHT; ++u) {
+        if (s0 + u >= deg) break;
+        const double s = (inf[u] & TILE_SECOND) ? dt : -dt;
+        sm.add(s * static_cast<double>(fl[u].a));
+        sE.add(s * static_cast<double>(fl[u].b));
+        sx.add(s * static_cast<double>(fl[u].c));
+        sy.add(s * static_cast<double>(fl[u].d));
+      }
+    }
+    double em = sm.value(), eE = sE.value(), ex = sx.value(), ey = sy.value();
+    if (a.flags & UPD_AVERAGE) {
+      em = 0.5 * (em + av.a);
+      eE = 0.5 * (eE + av.b);
+      ex = (ex + av.c) * 0.5;
+      ey = (ey + av.d) * 0.5;
+    }
+    a.out[i] = D4{em, eE, ex, ey};
+    // FP64 primitive recovery
+    const double d = em / vol;
+    const double vx = ex / em, vy = ey / em;
+    const double e = eE / em - 0.5 * (vx * vx + vy * vy);
+    if (d < 0)
+      report(erru, i, ERR_NEGATIVE_DENSITY);
+    else if (e < 0)
+      report(erru, i, ERR_NEGATIVE_THERMAL);
+    const double p = (prm.gamma - 1) * e * d;
+    a.cells[i] = F4{static_cast<float>(d), static_cast<float>(p), static_cast<float>(vx),
+                    static_cast<float>(vy)};
+    if (a.flags & UPD_CFL) cfl_v = cfl_inv(m, i, prm.gamma, wid, Cell{d, p, vx, vy});
+  }
+  if (a.flags & UPD_CFL) block_min(cfl_v, a.partials);
+}
+
+// The tiles' per-entry face data in entry order (set-up): the FP64 frame
+// or the FP32 face record, whichever is requested.
+__global__ void __launch_bounds__(256) k_tile_frames(DMesh m, MTables t,
+                                                     const int* __restrict__ tface,
+                                                     FaceFrame* __restrict__ tframe,
+                                                     FaceM* __restrict__ tfm, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int f = tface[e];
+  if (tframe) tframe[e] = m.fframe[f];
+  if (tfm) tfm[e] = t.face[f];
 }
 
 __global__ void k_to_f32(const D4* __restrict__ s, F4* __restrict__ d, int n) {

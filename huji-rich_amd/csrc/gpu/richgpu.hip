@@ -47,6 +47,7 @@ enum KernelId : int {
   KID_FLUX_PCM,
   KID_RIEMANN,
   KID_FACE_UPDATE,
+  KID_FACE_UPDATE_MIXED,
   KID_COUNT
 };
 const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_flux2",
@@ -56,7 +57,7 @@ const char* const kKernelNames[KID_COUNT] = {"k_cfl",         "k_slopes",   "k_f
                                              "k_flux_mixed",  "k_update_mixed", "k_adj_flux",
                                              "k_adj_slopes",  "k_adj_cell",     "k_ens",
                                              "k_moving",      "k_flux_pcm",     "k_riemann",
-                                             "k_face_update"};
+                                             "k_face_update", "k_face_update_mixed"};
 
 const char* err_message(int code) {
   switch (code) {
@@ -187,6 +188,7 @@ struct rg_ctx {
     std::vector<int> gsrc;       // [G]
     std::vector<int> wofs;       // [waves+1]
     std::vector<unsigned char> lanepos;  // [N] ELL column of each cell
+    std::vector<unsigned> minfo;  // [nnz] face words of the mixed update (RG_MIXED)
   } adj_src;
 
   // moving mesh: point velocities kept across rg_replace_mesh, and where
@@ -660,107 +662,142 @@ hipError_t heun_step(rg_ctx* c, int s, double* dt_hist, int hist, const HeunBufs
   return run_reduce_min(c, nxt);
 }
 
-// The tiles of k_face_update (second-order FP64 contexts, built at the
-// first Heun step loop of a mesh from the host copy of its incidences) and
-// the second state buffers.  A face with both sides in one tile becomes an
-// entry at its first side; every other incidence of a real cell makes its
-// own entry.  Entries follow the tile's cells and their faces in ascending
-// order.  A mesh whose largest tile would hold more than kTileMaxEntries
-// faces keeps the three-kernel stage (tiles_ok false).
+// Tile lists of the fused face-and-update kernels: for each tile of kTile
+// cells, the distinct faces its cells' slots read (ref(k): face word of
+// incidence k -- face | INFO_SECOND, the side whose sign the cell adds), in
+// the order of their first use (cells, then slots), and per ELL slot the
+// entry it reads.  Fails (false) where a tile would hold more than
+// kTileMaxEntries faces.
+struct TileLists {
+  std::vector<int> tofs, tface;
+  std::vector<unsigned short> eloc;
+  int max = 0;
+};
+
+template <class Ref>
+bool tile_lists(const rg_ctx::AdjSource& s, int N, int ne, Ref&& ref, TileLists& out) {
+  const int nt = (N + kTile - 1) / kTile;
+  std::vector<std::vector<int>> lists(static_cast<size_t>(nt));
+  out.eloc.assign(static_cast<size_t>(std::max(ne, 1)), 0);
+  bool bad = false;
+  rmp::parallel_for(nt, [&](int64_t tt) {
+    const int t = static_cast<int>(tt), i0 = t * kTile, i1 = std::min(N, i0 + kTile);
+    const int k0 = s.csr_off[i0], k1 = s.csr_off[i1];
+    // (face, first use) pairs, distinct faces, then their entries by first use
+    std::vector<std::pair<int, int>> fk;
+    fk.reserve(static_cast<size_t>(k1 - k0));
+    for (int k = k0; k < k1; ++k) fk.emplace_back(static_cast<int>(ref(k) & INFO_FACE), k);
+    std::sort(fk.begin(), fk.end());
+    std::vector<std::pair<int, int>> first;  // (first use, face)
+    for (size_t u = 0; u < fk.size(); ++u)
+      if (u == 0 || fk[u].first != fk[u - 1].first) first.emplace_back(fk[u].second, fk[u].first);
+    std::sort(first.begin(), first.end());
+    if (first.size() > static_cast<size_t>(kTileMaxEntries)) {
+      bad = true;
+      return;
+    }
+    std::vector<int>& faces = lists[static_cast<size_t>(t)];
+    faces.resize(first.size());
+    for (size_t u = 0; u < first.size(); ++u) faces[u] = first[u].second;
+    // entry of each face: position in `faces` (binary search on a sorted copy)
+    std::vector<std::pair<int, int>> pos(first.size());
+    for (size_t u = 0; u < first.size(); ++u) pos[u] = {faces[u], static_cast<int>(u)};
+    std::sort(pos.begin(), pos.end());
+    for (int i = i0; i < i1; ++i) {
+      const size_t base = static_cast<size_t>(s.wofs[i / 64] + s.lanepos[static_cast<size_t>(i)]);
+      for (int k = s.csr_off[i]; k < s.csr_off[i + 1]; ++k) {
+        const unsigned w = ref(k);
+        const int f = static_cast<int>(w & INFO_FACE);
+        const auto it = std::lower_bound(pos.begin(), pos.end(), std::make_pair(f, -1));
+        out.eloc[base + 64 * static_cast<size_t>(k - s.csr_off[i])] = static_cast<unsigned short>(
+            static_cast<unsigned>(it->second) | ((w & INFO_SECOND) ? TILE_SECOND : 0u));
+      }
+    }
+  }, 64);
+  if (bad) return false;
+  out.tofs.assign(static_cast<size_t>(nt) + 1, 0);
+  out.max = 0;
+  for (int t = 0; t < nt; ++t) {
+    const int n = static_cast<int>(lists[static_cast<size_t>(t)].size());
+    out.tofs[static_cast<size_t>(t) + 1] = out.tofs[t] + n;
+    out.max = std::max(out.max, n);
+  }
+  out.tface.resize(static_cast<size_t>(std::max(out.tofs[nt], 1)));
+  rmp::parallel_for(nt, [&](int64_t t) {
+    std::copy(lists[static_cast<size_t>(t)].begin(), lists[static_cast<size_t>(t)].end(),
+              out.tface.begin() + out.tofs[static_cast<size_t>(t)]);
+  }, 256);
+  return true;
+}
+
+template <class T>
+hipError_t tile_alloc(rg_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  const hipError_t e = hipMalloc(&q, sizeof(T) * std::max<size_t>(n, 1));
+  if (e == hipSuccess) c->tile_bufs.push_back(q);
+  *p = static_cast<T*>(q);
+  return e;
+}
+
+template <class T>
+hipError_t tile_upload(rg_ctx* c, T** p, const std::vector<T>& v) {
+  hipError_t e = tile_alloc(c, p, v.size());
+  if (e == hipSuccess && !v.empty())
+    e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream);
+  return e;
+}
+
+// The tiles of the fused face-and-update kernels and the second state
+// buffers of the FP64 loop, built at the first Heun step loop of a mesh
+// (second order): per entry the face's id, sides and frame (FP64) or FP32
+// face record (mixed, whose slots read a periodic boundary face through its
+// lower-numbered copy).  A mesh whose largest tile would hold more than
+// kTileMaxEntries faces keeps the three-kernel stage (tiles_ok false).
 hipError_t build_tiles(rg_ctx* c) {
   c->tiles_built = true;
   c->tiles_ok = false;
   const auto& s = c->adj_src;
-  const int N = c->N, nt = (N + kTile - 1) / kTile;
-  if (s.inc.size() != static_cast<size_t>(c->nnz) || s.csr_off.size() != static_cast<size_t>(N) + 1)
+  const int N = c->N;
+  const bool mixed = c->prm.precision == RG_MIXED;
+  if (s.inc.size() != static_cast<size_t>(c->nnz) || s.csr_off.size() != static_cast<size_t>(N) + 1 ||
+      (mixed && s.minfo.size() != static_cast<size_t>(c->nnz)))
     return hipSuccess;
-  std::vector<int> tofs(static_cast<size_t>(nt) + 1, 0);
-  std::vector<int> loc(static_cast<size_t>(std::max(c->nnz, 1)), -1);
-  auto inside = [&](int j, int t) { return j < N && j / kTile == t; };
-  // owners: their entry within the tile
-  rmp::parallel_for(nt, [&](int64_t tt) {
-    const int t = static_cast<int>(tt), i1 = std::min(N, (t + 1) * kTile);
-    int n = 0;
-    for (int i = t * kTile; i < i1; ++i)
-      for (int k = s.csr_off[i]; k < s.csr_off[i + 1]; ++k)
-        if (!inside(s.inc[k].nbr, t) || !(s.inc[k].info & INFO_SECOND)) loc[k] = n++;
-    tofs[static_cast<size_t>(t) + 1] = n;
-  }, 64);
-  int mx = 0;
-  for (int t = 0; t < nt; ++t) mx = std::max(mx, tofs[static_cast<size_t>(t) + 1]);
-  if (mx > kTileMaxEntries) return hipSuccess;
-  for (int t = 0; t < nt; ++t) tofs[static_cast<size_t>(t) + 1] += tofs[t];
-  const int total = tofs[nt];
-  std::vector<int> tface(static_cast<size_t>(std::max(total, 1)), 0);
-  std::vector<FaceSides> tside(static_cast<size_t>(std::max(total, 1)), FaceSides{0, 0});
-  std::vector<unsigned short> eloc(static_cast<size_t>(std::max(c->dm_ne, 1)), 0);
-  bool bad = false;
-  rmp::parallel_for(nt, [&](int64_t tt) {
-    const int t = static_cast<int>(tt), i1 = std::min(N, (t + 1) * kTile);
-    for (int i = t * kTile; i < i1; ++i) {
-      const size_t base = static_cast<size_t>(s.wofs[i / 64] + s.lanepos[static_cast<size_t>(i)]);
-      for (int k = s.csr_off[i]; k < s.csr_off[i + 1]; ++k) {
-        const unsigned info = s.inc[k].info;
-        const int f = static_cast<int>(info & INFO_FACE), j = s.inc[k].nbr;
-        int l = loc[k];
-        if (l < 0) {
-          // second side of a face inside the tile: the first side's entry
-          for (int kk = s.csr_off[j]; kk < s.csr_off[j + 1]; ++kk)
-            if (static_cast<int>(s.inc[kk].info & INFO_FACE) == f) l = loc[kk];
-          if (l < 0) {
-            bad = true;  // a face listed by one of its sides only
-            return;
-          }
-        } else {
-          const size_t e = static_cast<size_t>(tofs[t] + l);
-          tface[e] = f;
-          tside[e] = FaceSides{s.ff[f], s.fs[f]};
-        }
-        eloc[base + 64 * static_cast<size_t>(k - s.csr_off[i])] = static_cast<unsigned short>(
-            static_cast<unsigned>(l) | ((info & INFO_SECOND) ? TILE_SECOND : 0u));
-      }
-    }
-  }, 64);
-  if (bad) return hipSuccess;
-  auto get = [&](auto** p, size_t n) {
-    void* q = nullptr;
-    const hipError_t e = hipMalloc(&q, sizeof(**p) * std::max<size_t>(n, 1));
-    if (e == hipSuccess) c->tile_bufs.push_back(q);
-    *p = static_cast<std::remove_reference_t<decltype(*p)>>(q);
-    return e;
-  };
+  TileLists L;
+  const bool ok = mixed ? tile_lists(s, N, c->dm_ne, [&](int k) { return s.minfo[k]; }, L)
+                        : tile_lists(s, N, c->dm_ne, [&](int k) { return s.inc[k].info; }, L);
+  if (!ok) return hipSuccess;
+  const int total = L.tofs.back();
+  std::vector<FaceSides> tside(L.tface.size());
+  rmp::parallel_for(static_cast<int64_t>(L.tface.size()), [&](int64_t e) {
+    const int f = L.tface[static_cast<size_t>(e)];
+    tside[static_cast<size_t>(e)] = FaceSides{s.ff[f], s.fs[f]};
+  });
   int *tofs_d, *tface_d;
   FaceSides* tside_d;
-  FaceFrame* tframe_d;
   unsigned short* eloc_d;
-  hipError_t e = get(&tofs_d, tofs.size());
-  if (e == hipSuccess) e = get(&tface_d, tface.size());
-  if (e == hipSuccess) e = get(&tside_d, tside.size());
-  if (e == hipSuccess) e = get(&tframe_d, tface.size());
-  if (e == hipSuccess) e = get(&eloc_d, eloc.size());
-  if (e == hipSuccess) e = get(&c->alt_prim, static_cast<size_t>(N));
-  if (e == hipSuccess) e = get(&c->alt_ext, static_cast<size_t>(N));
-  if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(tofs_d, tofs.data(), sizeof(int) * tofs.size(), hipMemcpyHostToDevice,
-                     c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(tface_d, tface.data(), sizeof(int) * tface.size(), hipMemcpyHostToDevice,
-                       c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(tside_d, tside.data(), sizeof(FaceSides) * tside.size(),
-                       hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(eloc_d, eloc.data(), sizeof(unsigned short) * eloc.size(),
-                       hipMemcpyHostToDevice, c->stream);
+  FaceFrame* tframe_d = nullptr;
+  FaceM* tfm_d = nullptr;
+  hipError_t e = tile_upload(c, &tofs_d, L.tofs);
+  if (e == hipSuccess) e = tile_upload(c, &tface_d, L.tface);
+  if (e == hipSuccess) e = tile_upload(c, &tside_d, tside);
+  if (e == hipSuccess) e = tile_upload(c, &eloc_d, L.eloc);
+  if (e == hipSuccess) {
+    if (mixed)
+      e = tile_alloc(c, &tfm_d, L.tface.size());
+    else
+      e = tile_alloc(c, &tframe_d, L.tface.size());
+  }
+  if (e == hipSuccess && !mixed) e = tile_alloc(c, &c->alt_prim, static_cast<size_t>(N));
+  if (e == hipSuccess && !mixed) e = tile_alloc(c, &c->alt_ext, static_cast<size_t>(N));
   if (e != hipSuccess) return e;
   e = launch(c, KID_GEO, [&] {
-    k_tile_frames<<<blocks_for(std::max(total, 1)), kBlock, 0, c->stream>>>(c->dm, tface_d,
-                                                                           tframe_d, total);
+    k_tile_frames<<<blocks_for(std::max(total, 1)), kBlock, 0, c->stream>>>(
+        c->dm, c->mt, tface_d, tframe_d, tfm_d, total);
   });
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // the host vectors are uploaded
   if (e != hipSuccess) return e;
-  c->tiles = DTiles{tofs_d, tface_d, tside_d, tframe_d, eloc_d};
-  c->tiles_max = mx;
+  c->tiles = DTiles{tofs_d, tface_d, tside_d, tframe_d, tfm_d, eloc_d};
+  c->tiles_max = L.max;
   c->tiles_ok = true;
   return hipSuccess;
 }
@@ -956,6 +993,10 @@ hipError_t build_adjoint_tables(rg_ctx* c) {
 // extensives (RG_MIXED).  The FP64 cells are recovered at the end.
 int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
   const int N = c->N, F = c->F;
+  // each stage's faces and update fused per tile where the mesh has tiles
+  if (!c->tiles_built) HIPCHK(c, build_tiles(c));
+  const bool fused = c->tiles_ok;
+  const size_t lds = sizeof(F4) * static_cast<size_t>(c->tiles_max);
   HIPCHK(c, launch(c, KID_PACK, [&] {
     k_to_f32<<<blocks_for(N), kBlock, 0, c->stream>>>(c->prim, c->primM, N);
   }));
@@ -968,13 +1009,15 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
       const F4* q = stage == 0 ? c->primM : c->mid_primM;
       HIPCHK(c, launch(c, KID_SLOPES_MIXED, [&] {
         k_slopes_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(
-            c->dm, c->mt, q, c->recM, dparams_walk(c, stage), derr(c, s, 1 + 3 * stage));
+            c->dm, c->mt, q, c->recM, dparams_walk(c, fused ? 0 : stage),
+            derr(c, s, 1 + 3 * stage));
       }));
-      HIPCHK(c, launch(c, KID_FLUX_MIXED, [&] {
-        k_flux_mixed<<<blocks_for(F), kBlock, 0, c->stream>>>(
-            c->dm, c->mt, q, c->recM, c->fluxM, dparams_walk(c, 1 - stage),
-            derr(c, s, 2 + 3 * stage));
-      }));
+      if (!fused)
+        HIPCHK(c, launch(c, KID_FLUX_MIXED, [&] {
+          k_flux_mixed<<<blocks_for(F), kBlock, 0, c->stream>>>(
+              c->dm, c->mt, q, c->recM, c->fluxM, dparams_walk(c, 1 - stage),
+              derr(c, s, 2 + 3 * stage));
+        }));
       UpdArgsM a{};
       a.flux = c->fluxM;
       a.dt_slot = c->words + cur;
@@ -994,11 +1037,18 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
         a.partials = c->partials;
         a.flags = UPD_AVERAGE | UPD_CFL;
       }
-      HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
-        k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, a,
-                                                                dparams_walk(c, stage),
-                                                                derr(c, s, 3 + 3 * stage));
-      }));
+      if (fused)
+        HIPCHK(c, launch(c, KID_FACE_UPDATE_MIXED, [&] {
+          k_face_update_mixed<<<blocks_for(N), kTile, lds, c->stream>>>(
+              c->dm, c->mt, c->tiles, q, c->recM, a, dparams_walk(c, 1),
+              derr(c, s, 2 + 3 * stage), derr(c, s, 3 + 3 * stage));
+        }));
+      else
+        HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
+          k_update_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(c->dm, c->mt, a,
+                                                                  dparams_walk(c, stage),
+                                                                  derr(c, s, 3 + 3 * stage));
+        }));
     }
     HIPCHK(c, run_reduce_min(c, nxt));
   }
@@ -1539,6 +1589,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
     std::vector<unsigned> einfoM(ne, 0u);
     for (int i = 0; i < N; ++i)
       for (int k = m->csr_off[i]; k < m->csr_off[i + 1]; ++k) einfoM[ell_at(i, k)] = minfo[k];
+    c->adj_src.minfo = std::move(minfo);  // the mixed tiles' face words
     FaceM* faceM;
     CellGeoM* cgeoM;
     float2 *deh, *deo;
