@@ -111,11 +111,14 @@ def test_invalid_wave_speed_entries(order):
     g.close()
 
 
-def test_failed_step_keeps_the_state_and_reports_entries():
+@pytest.mark.parametrize("nsteps", [1, 2, 3])
+def test_failed_step_keeps_the_state_and_reports_entries(nsteps):
     """rg_advance with a cell whose energy cannot hold its kinetic energy:
     the stage-1 cell update of step 0 fails; the error names step 0, phase
     3 and carries that cell's mid-stage extensives (the oracle's), and the
-    context's cells and extensives are the ones the step started from."""
+    context's cells and extensives are the ones the step started from.  The
+    fused step loop alternates two state buffers, so odd and even step
+    counts end on different buffers."""
     from rich_amd.gpu import RichGPU, RichGPUError
     m = _mesh()
     cells = _cells(m, seed=10)
@@ -127,7 +130,7 @@ def test_failed_step_keeps_the_state_and_reports_entries():
     g.set_cells(cells)
     g.set_extensives(ext)
     with pytest.raises(RichGPUError) as e:
-        g.advance(3, integrator=2)
+        g.advance(nsteps, integrator=2)
     assert e.value.text == "Negative thermal energy"
     assert (e.value.step, e.value.phase, e.value.index) == (0, 3, 55)
     dt = o.cfl(cells)

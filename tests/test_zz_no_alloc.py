@@ -1,8 +1,9 @@
 """rg_advance_noalloc (the entry point that allocates no device memory)
 against rg_advance through the C ABI: two calls in a row on different
 contexts, the second on a smaller mesh with fewer steps, then rg_advance
-again on a context that used the no-alloc entry point; every dt, cell and
-extensive bit-identical.  Sizes beyond the scratch are refused."""
+again on a context that used the no-alloc entry point, and the no-alloc
+entry point again once rg_advance has built the fused stages' tiles; every
+dt, cell and extensive bit-identical.  Sizes beyond the scratch are refused."""
 import ctypes
 
 import numpy as np
@@ -44,6 +45,14 @@ def test_noalloc_equals_advance_twice_smaller_second():
         # the context's own buffers are back in use afterwards
         _same(new.advance(5, integrator=integrator), ref.advance(5, integrator=integrator), "dt")
         _same(new.get_cells(), ref.get_cells(), "cells after rg_advance")
+        # once rg_advance has built the fused stages' tiles, the no-alloc
+        # entry point runs them too (odd and even step counts: the state
+        # alternates between two buffers)
+        for k in (7, 4):
+            _same(new.advance_noalloc(k, integrator=integrator),
+                  ref.advance(k, integrator=integrator), "dt (tiles built)")
+            _same(new.get_cells(), ref.get_cells(), "cells (tiles built)")
+            _same(new.get_extensives(), ref.get_extensives(), "extensives (tiles built)")
         ref.close()
         new.close()
 
