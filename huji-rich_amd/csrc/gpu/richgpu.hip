@@ -2005,6 +2005,10 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
   }
   const size_t bytes = sizeof(D4) * static_cast<size_t>(N);
   c->ext_init = false;
+  // the fused stages where the mesh has tiles, alternating the two state
+  // buffers as rg_advance does
+  if (!c->tiles_built) HIPCHK(c, build_tiles(c));
+  const bool fused = c->tiles_ok;
   HIPCHK(c, run_cfl(c, c->prim, 0));
   for (int s = 0; s < nsteps; ++s) {
     if (s % every == 0) {
@@ -2012,14 +2016,22 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
       HIPCHK(c, hipMemcpyAsync(c->ckP + k, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(c, hipMemcpyAsync(c->ckX + k, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
     }
-    HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s, ctx_bufs(c)));
+    if (fused) {
+      const HeunBufs h{c->prim, c->ext, c->mid_prim, c->mid_ext,
+                       c->alt_prim, c->alt_ext, c->rec, c->rec};
+      HIPCHK(c, heun_step_fused(c, s, dt_out ? c->dt_hist : nullptr, s, h));
+      std::swap(c->prim, c->alt_prim);
+      std::swap(c->ext, c->alt_ext);
+    } else {
+      HIPCHK(c, heun_step(c, s, dt_out ? c->dt_hist : nullptr, s, ctx_bufs(c)));
+    }
   }
   HIPCHK(c, hipMemcpyAsync(c->finP, c->prim, bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->finX, c->ext, bytes, hipMemcpyDeviceToDevice, c->stream));
   if (dt_out)
     HIPCHK(c, hipMemcpyAsync(dt_out, c->dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost,
                              c->stream));
-  const int rc = collect_error(c);
+  const int rc = collect_fp64(c, 2, nsteps, fused);
   if (rc != RG_OK) return rc;
   c->adj_steps = nsteps;
   c->adj_every = every;
@@ -2112,7 +2124,9 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
       const size_t o = static_cast<size_t>(t) * n;
       const HeunBufs h{T.P + o,     T.X + o,     T.Pm + o, T.Xm + o,
                        T.P + o + n, T.X + o + n, T.R1 + o, T.R2 + o};
-      e = heun_step(c, k0 + t, T.dt, t, h);
+      // each step's end state goes to the next tape slot, as the fused
+      // stages require
+      e = c->tiles_ok ? heun_step_fused(c, k0 + t, T.dt, t, h) : heun_step(c, k0 + t, T.dt, t, h);
     }
     if (e == hipSuccess) e = hipEventRecord(c->tape_ready[seg & 1], c->stream);
     return e;
