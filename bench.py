@@ -199,9 +199,9 @@ def moving_leg(args):
 
 def ensemble_leg(args):
     """256 independent Sedov runs of uneven size advanced together (one
-    hipGraph launch of persistent kernels per step, per-instance dt, launch
-    shape from the autotuner), against the same instances advanced one at a
-    time."""
+    hipGraph launch per step, per-instance dt, the slope kernel's launch
+    shape from the autotuner, faces and cell update fused per tile), against
+    the same instances advanced one at a time."""
     import numpy as np
     from rich_amd import ensemble
     from rich_amd.gpu import RichGPU
@@ -226,6 +226,7 @@ def ensemble_leg(args):
     dts = g.advance_ensemble(steps)
     el = time.perf_counter() - t
     final = u.split(g.get_cells())
+    fused, entries, _ = g.tiles()
     g.close()
     N = int(u.cell_off[-1])
     # a sample advanced alone: the same bits, and the sequential rate
@@ -254,7 +255,8 @@ def ensemble_leg(args):
             "steps": steps, "value": value, "unit": "Mcell-updates/s",
             "ms_per_step": 1e3 * el / steps,
             "launch": {"block": best[0], "blocks_per_cu": best[1], "graph_launches_per_step": 1,
-                       "kernels_per_step": 8},
+                       "kernels_per_step": 6 if fused else 8,
+                       "fused_stages": fused, "tile_entries": entries},
             "autotune_ms_per_step": [[b, p, round(t_, 4)] for b, p, t_ in table],
             "one_at_a_time": {"instances": len(sample), "value": seq_rate,
                               "batched_speedup": value / seq_rate},
@@ -565,14 +567,17 @@ def main():
                             "traffic": sum(traffic) if None not in traffic else None,
                             "alg_bytes_per_launch": b_alg, "avg_launch_us": stage_s * 1e6,
                             "alg_bytes": "180 N + 48 F per cell-stage"}
-        ent = tile_entries(mesh)
+        fused, ent, _ = g.tiles()
+        if not fused:
+            ent = tile_entries(mesh)
         line["kernels"] = {k: {"launches": n, "avg_us": 1e3 * ms / n,
                                "model_bytes": kernel_alg_bytes(k, N, F, G, nnz, entries=ent),
                                "model_GBs": kernel_alg_bytes(k, N, F, G, nnz, entries=ent)
                                / (ms / n / 1e3) / 1e9,
                                "traffic": measured_traffic(k, args.nx, args.ny)}
                            for k, (n, ms) in sorted(prof.items())}
-        line["tile_entries"] = {"entries": ent, "per_face": ent / F, "cells_per_tile": TILE}
+        line["tile_entries"] = {"fused_stages": fused, "entries": ent, "per_face": ent / F,
+                                "cells_per_tile": TILE}
     stage_gbs = b_alg * 2 * args.steps / elapsed / 1e9
     line["stage_roofline"] = {"bytes_per_stage": b_alg, "achieved": stage_gbs,
                               "frac": stage_gbs / HBM_PEAK_GBS, "unit": "GB/s"}

@@ -2,8 +2,8 @@
 // context's mesh is the disjoint union of the instances' meshes (cells of
 // instance k are coff[k]..coff[k+1]-1), so the cell and face kernels serve
 // every instance in one launch whatever their sizes; only the time step is
-// per instance.  A step is eight persistent launches captured once into a
-// hipGraph and replayed per step; every quantity that changes between steps
+// per instance.  A step is eight launches (six where the stages run fused
+// per tile) captured once into a hipGraph and replayed per step; every quantity that changes between steps
 // (the step index, the dt slots) lives in device memory, so the graph is the
 // same for every step.
 #pragma once
@@ -32,6 +32,64 @@ __global__ void __launch_bounds__(256) k_update_ens_p(DMesh m, UpdArgs a, Ens e,
     const double dt =
         prm.cfl * __longlong_as_double(static_cast<long long>(e.dtA[e.inst[i]]));
     const double v = update_cell(m, a, prm, err, i, dt);
+    if (a.flags & UPD_CFL) e.inv[i] = v;
+  }
+}
+
+// One stage of every instance fused per tile of kTile cells (k_face_update
+// with per-instance time steps): each face's dt*area*F with its instance's
+// dt (both sides of a face belong to one instance) into LDS, then each cell's
+// update with its own instance's dt and its CFL inverse into e.inv.
+__global__ void __launch_bounds__(256) k_face_update_ens(DMesh m, DTiles tl,
+                                                         const D4* __restrict__ q,
+                                                         const Rec* __restrict__ rec, UpdArgs a,
+                                                         Ens e, DParams prm, DErr errf, DErr erru) {
+  extern __shared__ D4 tflux[];
+  const int t = walk_block(prm);
+  const int e0 = tl.tofs[t], ne = tl.tofs[t + 1] - e0;
+  FaceFrame frn{0, 0, 0, 0, 0, 0};
+  FaceSides sdn{0, 0};
+  if (static_cast<int>(threadIdx.x) < ne) {
+    frn = tl.tframe[e0 + threadIdx.x];
+    sdn = tl.tside[e0 + threadIdx.x];
+  }
+  for (int k = threadIdx.x; k < ne; k += blockDim.x) {
+    const int ent = e0 + k;
+    const FaceFrame fr = frn;
+    const FaceSides sd = sdn;
+    if (k + static_cast<int>(blockDim.x) < ne) {
+      frn = tl.tframe[ent + blockDim.x];
+      sdn = tl.tside[ent + blockDim.x];
+    }
+    const int real = sd.a < m.N ? sd.a : sd.b;
+    const double dt =
+        prm.cfl * __longlong_as_double(static_cast<long long>(e.dtA[e.inst[real]]));
+    Cell lf, rt;
+    face_states(m, q, rec, sd.a, sd.b, fr, lf, rt);
+    const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
+    // modular_flux_calculator.cpp:56-67 (an ensemble has no face velocities)
+    const double speed = (fr.nx * 0.0 + fr.ny * 0.0) / an;
+    Cons c;
+    const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
+    D4 o{0, 0, 0, 0};
+    if (rc) {
+      report(errf, tl.tface[ent], rc);
+    } else {
+      const double sc = dt * ap;  // SimpleExtensiveUpdater's dt*area
+      o = D4{c.m * sc, c.E * sc, c.mx * sc, c.my * sc};
+    }
+    tflux[k] = o;
+  }
+  const int i = t * kTile + threadIdx.x;
+  UpdLoads L{};
+  double dt = 0;
+  if (i < m.N) {
+    L = update_loads<true>(m, a, erru, i, tl.eloc);
+    dt = prm.cfl * __longlong_as_double(static_cast<long long>(e.dtA[e.inst[i]]));
+  }
+  __syncthreads();
+  if (i < m.N) {
+    const double v = update_cell<true, true>(m, a, prm, erru, i, dt, tl.eloc, tflux, &L);
     if (a.flags & UPD_CFL) e.inv[i] = v;
   }
 }

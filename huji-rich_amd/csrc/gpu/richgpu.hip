@@ -202,6 +202,7 @@ struct rg_ctx {
   // second state buffers that loop alternates with
   DTiles tiles{};
   int tiles_max = 0;
+  int64_t tiles_total = 0;
   bool tiles_built = false, tiles_ok = false;
   std::vector<void*> tile_bufs;
   D4 *alt_prim = nullptr, *alt_ext = nullptr;
@@ -305,6 +306,7 @@ void free_mesh(rg_ctx* c, bool pool = false) {
   c->tile_bufs.clear();
   c->tiles = DTiles{};
   c->tiles_max = 0;
+  c->tiles_total = 0;
   c->tiles_built = c->tiles_ok = false;
   c->alt_prim = c->alt_ext = nullptr;
   c->am = AdjMesh{};
@@ -798,6 +800,7 @@ hipError_t build_tiles(rg_ctx* c) {
   if (e != hipSuccess) return e;
   c->tiles = DTiles{tofs_d, tface_d, tside_d, tframe_d, tfm_d, eloc_d};
   c->tiles_max = L.max;
+  c->tiles_total = total;
   c->tiles_ok = true;
   return hipSuccess;
 }
@@ -891,12 +894,27 @@ hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
   b.cells = c->prim;
   b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   const hipStream_t st = c->stream;
-  k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
-  k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->prim, c->rec, c->flux, prm, derr(c, 0, 2));
-  k_update_ens_p<<<grid, block, 0, st>>>(c->dm, a, e, prm, derr(c, 0, 3));
-  k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, prm, derr(c, 0, 4));
-  k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->mid_prim, c->rec, c->flux, prm, derr(c, 0, 5));
-  k_update_ens_p<<<grid, block, 0, st>>>(c->dm, b, e, prm, derr(c, 0, 6));
+  if (c->tiles_ok) {
+    // each stage's faces and update fused per tile; the slope kernel keeps
+    // the tuned launch shape, the tile kernel walks its tiles in reverse
+    DParams pr = prm;
+    pr.rev = 1;
+    const size_t lds = sizeof(D4) * static_cast<size_t>(c->tiles_max);
+    const unsigned nt = blocks_for(c->N);
+    k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
+    k_face_update_ens<<<nt, kTile, lds, st>>>(c->dm, c->tiles, c->prim, c->rec, a, e, pr,
+                                              derr(c, 0, 2), derr(c, 0, 3));
+    k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, prm, derr(c, 0, 4));
+    k_face_update_ens<<<nt, kTile, lds, st>>>(c->dm, c->tiles, c->mid_prim, c->rec, b, e, pr,
+                                              derr(c, 0, 5), derr(c, 0, 6));
+  } else {
+    k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
+    k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->prim, c->rec, c->flux, prm, derr(c, 0, 2));
+    k_update_ens_p<<<grid, block, 0, st>>>(c->dm, a, e, prm, derr(c, 0, 3));
+    k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->mid_prim, c->rec, prm, derr(c, 0, 4));
+    k_flux2_p<<<gridF, block, 0, st>>>(c->dm, c->mid_prim, c->rec, c->flux, prm, derr(c, 0, 5));
+    k_update_ens_p<<<grid, block, 0, st>>>(c->dm, b, e, prm, derr(c, 0, 6));
+  }
   k_ens_min<<<static_cast<unsigned>(std::min(e.K, 4096)), 256, 0, st>>>(e);
   k_ens_finish<<<1, 256, 0, st>>>(e, c->prm.cfl, 1);
   return hipGetLastError();
@@ -906,6 +924,11 @@ hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
 hipError_t ens_capture(rg_ctx* c, int block, int bpc) {
   if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
   c->ens_graph = nullptr;
+  // the tiles of the fused stages (allocated here, outside the capture)
+  if (!c->tiles_built) {
+    const hipError_t eb = build_tiles(c);
+    if (eb != hipSuccess) return eb;
+  }
   hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
   const hipError_t e1 = ens_record_step(c, block, bpc);
@@ -2548,5 +2571,12 @@ int rg_profile_read(rg_ctx* c, int32_t id, int64_t* launches, double* total_ms) 
 }
 
 int64_t rg_device_bytes(rg_ctx* c) { return c ? c->bytes : 0; }
+
+int32_t rg_tiles(rg_ctx* c, int64_t* entries, int32_t* max_entries) {
+  const bool on = c && c->tiles_ok;
+  if (entries) *entries = on ? c->tiles_total : 0;
+  if (max_entries) *max_entries = on ? c->tiles_max : 0;
+  return on ? 1 : 0;
+}
 
 }  // extern "C"

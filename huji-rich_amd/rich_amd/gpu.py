@@ -102,6 +102,8 @@ def lib():
     L.rg_profile_read.argtypes = [vp, ctypes.c_int32, P(ctypes.c_int64), P(ctypes.c_double)]
     L.rg_device_bytes.restype = ctypes.c_int64
     L.rg_device_bytes.argtypes = [vp]
+    L.rg_tiles.restype = ctypes.c_int32
+    L.rg_tiles.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int32)]
     L._typed = True
     return L
 
@@ -371,3 +373,11 @@ class RichGPU:
 
     def device_bytes(self):
         return self._lib.rg_device_bytes(self._h)
+
+    def tiles(self):
+        """(fused, face entries per stage, largest tile's entries): whether the
+        second-order Heun loops run their stages fused per tile of 256 cells."""
+        n = ctypes.c_int64()
+        mx = ctypes.c_int32()
+        on = self._lib.rg_tiles(self._h, ctypes.byref(n), ctypes.byref(mx))
+        return bool(on), n.value, mx.value

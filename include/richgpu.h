@@ -294,6 +294,13 @@ const char* rg_kernel_name(int32_t id);
 int rg_profile_read(rg_ctx* ctx, int32_t kernel_id, int64_t* launches, double* total_ms);
 /* Bytes of device memory the context holds. */
 int64_t rg_device_bytes(rg_ctx* ctx);
+/* Whether the second-order Heun loops (rg_advance, the ensemble graph, the
+ * adjoint) run each stage's faces and cell update fused per tile of 256
+ * cells: 1 once the mesh's tiles are built and fit, else 0.  *entries and
+ * *max_entries (either may be NULL) receive the faces solved per stage (a
+ * face between two tiles counts twice) and the largest tile's count.  No
+ * counterpart in the reference: a diagnostic of this library's layout. */
+int32_t rg_tiles(rg_ctx* ctx, int64_t* entries, int32_t* max_entries);
 
 #ifdef __cplusplus
 }
