@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(256) k_face_update_mixed(DMesh m, MTables t, D
     x = a.in[i];
     if (a.flags & UPD_AVERAGE) av = a.avg[i];
     vol = m.vol[i];
-    if (a.flags & UPD_CFL) wid = t.cgeo[i].width;
+    if (a.flags & UPD_CFL) wid = m.width[i];  // the FP64 width, as k_update_mixed reads it
     deg = m.off[i + 1] - m.off[i];
     eb = ell_base(t, i);
 #pragma unroll

@@ -204,6 +204,7 @@ struct rg_ctx {
   int tiles_max = 0;
   int64_t tiles_total = 0;
   bool tiles_built = false, tiles_ok = false;
+  bool tiles_on = true;  // rg_use_tiles: the fused stages are allowed
   std::vector<void*> tile_bufs;
   D4 *alt_prim = nullptr, *alt_ext = nullptr;
   double* riemann_buf = nullptr;  // rg_riemann's states and fluxes
@@ -755,6 +756,9 @@ hipError_t tile_upload(rg_ctx* c, T** p, const std::vector<T>& v) {
 // face record (mixed, whose slots read a periodic boundary face through its
 // lower-numbered copy).  A mesh whose largest tile would hold more than
 // kTileMaxEntries faces keeps the three-kernel stage (tiles_ok false).
+// The fused stages run: allowed (rg_use_tiles) and the mesh's tiles fit.
+bool fused_ok(const rg_ctx* c) { return c->tiles_on && c->tiles_ok; }
+
 hipError_t build_tiles(rg_ctx* c) {
   c->tiles_built = true;
   c->tiles_ok = false;
@@ -894,7 +898,7 @@ hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
   b.cells = c->prim;
   b.flags = UPD_FACES | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   const hipStream_t st = c->stream;
-  if (c->tiles_ok) {
+  if (fused_ok(c)) {
     // each stage's faces and update fused per tile; the slope kernel keeps
     // the tuned launch shape, the tile kernel walks its tiles in reverse
     DParams pr = prm;
@@ -925,7 +929,7 @@ hipError_t ens_capture(rg_ctx* c, int block, int bpc) {
   if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
   c->ens_graph = nullptr;
   // the tiles of the fused stages (allocated here, outside the capture)
-  if (!c->tiles_built) {
+  if (c->tiles_on && !c->tiles_built) {
     const hipError_t eb = build_tiles(c);
     if (eb != hipSuccess) return eb;
   }
@@ -1017,8 +1021,8 @@ hipError_t build_adjoint_tables(rg_ctx* c) {
 int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
   const int N = c->N, F = c->F;
   // each stage's faces and update fused per tile where the mesh has tiles
-  if (!c->tiles_built) HIPCHK(c, build_tiles(c));
-  const bool fused = c->tiles_ok;
+  if (c->tiles_on && !c->tiles_built) HIPCHK(c, build_tiles(c));
+  const bool fused = fused_ok(c);
   const size_t lds = sizeof(F4) * static_cast<size_t>(c->tiles_max);
   HIPCHK(c, launch(c, KID_PACK, [&] {
     k_to_f32<<<blocks_for(N), kBlock, 0, c->stream>>>(c->prim, c->primM, N);
@@ -1098,11 +1102,11 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
 hipError_t advance_fp64(rg_ctx* c, int integrator, int nsteps, bool record, bool may_build,
                         bool* fused) {
   const bool can_fuse = integrator == 2 && c->prm.order == 2;
-  if (can_fuse && may_build && !c->tiles_built) {
+  if (can_fuse && may_build && c->tiles_on && !c->tiles_built) {
     const hipError_t eb = build_tiles(c);
     if (eb != hipSuccess) return eb;
   }
-  *fused = can_fuse && c->tiles_ok;
+  *fused = can_fuse && fused_ok(c);
   // first dt: SimpleCFL on the current cells into slot 0
   hipError_t e = run_cfl(c, c->prim, 0);
   for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
@@ -2030,8 +2034,8 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
   c->ext_init = false;
   // the fused stages where the mesh has tiles, alternating the two state
   // buffers as rg_advance does
-  if (!c->tiles_built) HIPCHK(c, build_tiles(c));
-  const bool fused = c->tiles_ok;
+  if (c->tiles_on && !c->tiles_built) HIPCHK(c, build_tiles(c));
+  const bool fused = fused_ok(c);
   HIPCHK(c, run_cfl(c, c->prim, 0));
   for (int s = 0; s < nsteps; ++s) {
     if (s % every == 0) {
@@ -2149,7 +2153,7 @@ int rg_adjoint_backward(rg_ctx* c, const double* gcells, const double* gext, dou
                        T.P + o + n, T.X + o + n, T.R1 + o, T.R2 + o};
       // each step's end state goes to the next tape slot, as the fused
       // stages require
-      e = c->tiles_ok ? heun_step_fused(c, k0 + t, T.dt, t, h) : heun_step(c, k0 + t, T.dt, t, h);
+      e = fused_ok(c) ? heun_step_fused(c, k0 + t, T.dt, t, h) : heun_step(c, k0 + t, T.dt, t, h);
     }
     if (e == hipSuccess) e = hipEventRecord(c->tape_ready[seg & 1], c->stream);
     return e;
@@ -2572,8 +2576,19 @@ int rg_profile_read(rg_ctx* c, int32_t id, int64_t* launches, double* total_ms) 
 
 int64_t rg_device_bytes(rg_ctx* c) { return c ? c->bytes : 0; }
 
+int rg_use_tiles(rg_ctx* c, int32_t on) {
+  if (!c || (on != 0 && on != 1)) return RG_E_ARG;
+  if (c->tiles_on != (on == 1)) {
+    c->tiles_on = on == 1;
+    // the ensemble's graph holds the other stage form
+    if (c->ens_graph) (void)hipGraphExecDestroy(c->ens_graph);
+    c->ens_graph = nullptr;
+  }
+  return RG_OK;
+}
+
 int32_t rg_tiles(rg_ctx* c, int64_t* entries, int32_t* max_entries) {
-  const bool on = c && c->tiles_ok;
+  const bool on = c && fused_ok(c);
   if (entries) *entries = on ? c->tiles_total : 0;
   if (max_entries) *max_entries = on ? c->tiles_max : 0;
   return on ? 1 : 0;

@@ -104,6 +104,7 @@ def lib():
     L.rg_device_bytes.argtypes = [vp]
     L.rg_tiles.restype = ctypes.c_int32
     L.rg_tiles.argtypes = [vp, P(ctypes.c_int64), P(ctypes.c_int32)]
+    L.rg_use_tiles.argtypes = [vp, ctypes.c_int32]
     L._typed = True
     return L
 
@@ -373,6 +374,10 @@ class RichGPU:
 
     def device_bytes(self):
         return self._lib.rg_device_bytes(self._h)
+
+    def use_tiles(self, on=True):
+        """Allow (True, the default) or forbid the fused tile stages."""
+        self._check(self._lib.rg_use_tiles(self._h, 1 if on else 0))
 
     def tiles(self):
         """(fused, face entries per stage, largest tile's entries): whether the

@@ -301,6 +301,13 @@ int64_t rg_device_bytes(rg_ctx* ctx);
  * face between two tiles counts twice) and the largest tile's count.  No
  * counterpart in the reference: a diagnostic of this library's layout. */
 int32_t rg_tiles(rg_ctx* ctx, int64_t* entries, int32_t* max_entries);
+/* on = 1 (the default): the second-order Heun loops run the fused tile
+ * stages (built at their first call on a mesh); on = 0: they run the
+ * three-kernel stages (slopes, faces into a face array, cell update), with
+ * the same results bit for bit and no tile memory built.  No counterpart in
+ * the reference: for callers short of device memory, and for tests that
+ * compare the two forms. */
+int rg_use_tiles(rg_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus
 }

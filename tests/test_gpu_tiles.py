@@ -69,3 +69,77 @@ def test_mixed_context_builds_its_own_tiles():
     # boundary faces
     assert fused and abs(n - _entries(m)[0]) <= 2 * m.n_ghosts and 0 < mx <= 2040
     g.close()
+
+
+def _rough(m, seed=3):
+    cells = cases.rough_cells(m, seed=seed)
+    cells[0] = np.abs(cells[0]) + 0.5
+    cells[1] = np.abs(cells[1]) + 0.5
+    return cells
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_fused_and_three_kernel_stages_agree(mixed):
+    """rg_use_tiles(0) runs the three-kernel stages: the same bits as the
+    fused ones, in FP64 and in mixed precision, also when the switch is
+    flipped between two calls (the fused loop left the state in its second
+    buffers)."""
+    from rich_amd.gpu import RG_FP64, RG_MIXED, RichGPU
+    m = rmesh.lattice(40, 24, (0.0, 1.0, 0.0, 0.6), jitter=2e-2, seed=5, bc=rmesh.PERIODIC)
+    cells = _rough(m)
+    runs = []
+    for plan in ((True, True), (False, False), (True, False)):
+        g = RichGPU(m, order=2, precision=RG_MIXED if mixed else RG_FP64)
+        g.set_cells(cells)
+        g.init_extensives()
+        dts = []
+        for on in plan:
+            g.use_tiles(on)
+            dts.append(g.advance(3, integrator=2))
+            assert g.tiles()[0] == on
+        runs.append((np.concatenate(dts), g.get_cells(), g.get_extensives()))
+        g.close()
+    for got in runs[1:]:
+        for a, b in zip(got, runs[0]):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_ensemble_fused_and_three_kernel_stages_agree():
+    from rich_amd import ensemble
+    from rich_amd.gpu import RichGPU
+    ms, cells, _ = ensemble.sedov_ensemble(K=5, n_min=8, n_max=20, seed=3)
+    u = ensemble.union(ms)
+    runs = []
+    for on in (True, False):
+        g = RichGPU(u.mesh, order=2)
+        g.use_tiles(on)
+        g.set_cells(np.concatenate(cells, axis=1))
+        g.init_extensives()
+        g.set_instances(u.cell_off)
+        dts = g.advance_ensemble(8)
+        assert g.tiles()[0] == on and u.mesh.n_cells > 2 * TILE
+        runs.append((dts, g.get_cells(), g.get_extensives()))
+        g.close()
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_adjoint_fused_and_three_kernel_stages_agree():
+    """The checkpointing forward and the backward's recompute give the same
+    gradient bit for bit with either stage form."""
+    from rich_amd.gpu import RichGPU
+    m = cases.sedov_mesh(24)
+    cells = cases.sedov_cells(m, radius=0.15)
+    w = np.random.default_rng(7).standard_normal((4, m.n_cells))
+    runs = []
+    for on in (True, False):
+        g = RichGPU(m, order=2)
+        g.use_tiles(on)
+        g.set_cells(cells)
+        g.init_extensives()
+        dts = g.adjoint_forward(12, every=4)
+        assert g.tiles()[0] == on and m.n_cells > 2 * TILE
+        runs.append((dts, g.get_cells(), g.adjoint_backward(w)))
+        g.close()
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
