@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -682,7 +683,7 @@ bool tile_lists(const rg_ctx::AdjSource& s, int N, int ne, Ref&& ref, TileLists&
   const int nt = (N + kTile - 1) / kTile;
   std::vector<std::vector<int>> lists(static_cast<size_t>(nt));
   out.eloc.assign(static_cast<size_t>(std::max(ne, 1)), 0);
-  bool bad = false;
+  std::atomic<bool> bad{false};
   rmp::parallel_for(nt, [&](int64_t tt) {
     const int t = static_cast<int>(tt), i0 = t * kTile, i1 = std::min(N, i0 + kTile);
     const int k0 = s.csr_off[i0], k1 = s.csr_off[i1];
@@ -696,7 +697,7 @@ bool tile_lists(const rg_ctx::AdjSource& s, int N, int ne, Ref&& ref, TileLists&
       if (u == 0 || fk[u].first != fk[u - 1].first) first.emplace_back(fk[u].second, fk[u].first);
     std::sort(first.begin(), first.end());
     if (first.size() > static_cast<size_t>(kTileMaxEntries)) {
-      bad = true;
+      bad.store(true, std::memory_order_relaxed);
       return;
     }
     std::vector<int>& faces = lists[static_cast<size_t>(t)];
@@ -717,7 +718,7 @@ bool tile_lists(const rg_ctx::AdjSource& s, int N, int ne, Ref&& ref, TileLists&
       }
     }
   }, 64);
-  if (bad) return false;
+  if (bad.load()) return false;
   out.tofs.assign(static_cast<size_t>(nt) + 1, 0);
   out.max = 0;
   for (int t = 0; t < nt; ++t) {
