@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_update_ens_p(DMesh m, UpdArgs a, Ens e,
 // with per-instance time steps): each face's dt*area*F with its instance's
 // dt (both sides of a face belong to one instance) into LDS, then each cell's
 // update with its own instance's dt and its CFL inverse into e.inv.
-__global__ void __launch_bounds__(256) k_face_update_ens(DMesh m, DTiles tl,
+__global__ void __launch_bounds__(kTile) k_face_update_ens(DMesh m, DTiles tl,
                                                          const D4* __restrict__ q,
                                                          const Rec* __restrict__ rec, UpdArgs a,
                                                          Ens e, DParams prm, DErr errf, DErr erru) {

@@ -124,7 +124,10 @@ __device__ __forceinline__ int ell_base(const DMesh& m, int i) {
 // only for face velocities and errors.  Per ELL slot, eloc holds the entry of
 // the slot's face within the cell's tile, with TILE_SECOND where the cell is
 // the face's second side.
-constexpr int kTile = 256;
+#ifndef RG_TILE
+#define RG_TILE 256
+#endif
+constexpr int kTile = RG_TILE;  // cells per tile = threads per block of the tile kernels
 constexpr unsigned TILE_SECOND = 0x8000u;
 constexpr unsigned TILE_ENTRY = 0x7FFFu;
 constexpr int kTileMaxEntries = 2040;  // fluxes and block_min within 64 KiB of LDS
@@ -1135,7 +1138,7 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
 // is computed by both tiles, with equal results.  The stage must not write
 // what its faces read: the update's outputs (a.out, a.cells) are other
 // buffers than the cells q and slopes rec.
-__global__ void __launch_bounds__(256) k_face_update(DMesh m, DTiles tl, const D4* __restrict__ q,
+__global__ void __launch_bounds__(kTile) k_face_update(DMesh m, DTiles tl, const D4* __restrict__ q,
                                                      const Rec* __restrict__ rec, UpdArgs a,
                                                      DParams prm, DErr errf, DErr erru) {
   extern __shared__ D4 tflux[];

@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArg
 // k_update_mixed reads it) into LDS, then k_update_mixed's update of the
 // tile's cells from those fluxes.  The update's outputs are other buffers
 // than the cells and slopes the faces read.
-__global__ void __launch_bounds__(256) k_face_update_mixed(DMesh m, MTables t, DTiles tl,
+__global__ void __launch_bounds__(kTile) k_face_update_mixed(DMesh m, MTables t, DTiles tl,
                                                            const F4* __restrict__ q,
                                                            const RecM* __restrict__ rec,
                                                            UpdArgsM a, DParams prm, DErr errf,

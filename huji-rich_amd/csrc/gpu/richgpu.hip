@@ -391,10 +391,13 @@ DParams dparams_walk(const rg_ctx* c, int rev) {
 }
 
 DErr derr(rg_ctx* c, int step, int phase) { return DErr{c->words + 2, step, phase}; }
-hipError_t run_reduce_min(rg_ctx* c, int slot) {
+// The minimum of the CFL block minima into dt slot `slot`: one per block of
+// kBlock cells, or per tile (`tiles`) after a fused stage.
+inline unsigned tiles_for(int n) { return static_cast<unsigned>((n + kTile - 1) / kTile); }
+hipError_t run_reduce_min(rg_ctx* c, int slot, bool tiles = false) {
+  const int n = static_cast<int>(tiles ? tiles_for(c->N) : blocks_for(c->N));
   return launch(c, KID_REDUCE, [&] {
-    k_reduce_min<<<1, 1024, 0, c->stream>>>(c->partials, static_cast<int>(blocks_for(c->N)),
-                                            c->words + slot);
+    k_reduce_min<<<1, 1024, 0, c->stream>>>(c->partials, n, c->words + slot);
   });
 }
 
@@ -819,7 +822,7 @@ hipError_t fused_stage(rg_ctx* c, const D4* cells, Rec* r, const UpdArgs& a, int
   if (e != hipSuccess) return e;
   const size_t lds = sizeof(D4) * static_cast<size_t>(c->tiles_max);
   return launch(c, KID_FACE_UPDATE, [&] {
-    k_face_update<<<blocks_for(c->N), kTile, lds, c->stream>>>(
+    k_face_update<<<tiles_for(c->N), kTile, lds, c->stream>>>(
         c->dm, c->tiles, cells, r, a, dparams_walk(c, 1), derr(c, s, phase_base + 1),
         derr(c, s, phase_base + 2));
   });
@@ -852,7 +855,7 @@ hipError_t heun_step_fused(rg_ctx* c, int s, double* dt_hist, int hist, const He
   b.flags = UPD_FACES | UPD_DELTA | UPD_ZF_BEFORE | UPD_AVERAGE | UPD_CELLS | UPD_CFL;
   e = fused_stage(c, h.Pm, h.r2, b, s, 4);
   if (e != hipSuccess) return e;
-  return run_reduce_min(c, nxt);
+  return run_reduce_min(c, nxt, true);
 }
 
 Ens ens_args(rg_ctx* c) {
@@ -905,7 +908,7 @@ hipError_t ens_record_step(rg_ctx* c, int block, int bpc) {
     DParams pr = prm;
     pr.rev = 1;
     const size_t lds = sizeof(D4) * static_cast<size_t>(c->tiles_max);
-    const unsigned nt = blocks_for(c->N);
+    const unsigned nt = tiles_for(c->N);
     k_slopes_p<<<grid, block, 0, st>>>(c->dm, c->prim, c->rec, prm, derr(c, 0, 1));
     k_face_update_ens<<<nt, kTile, lds, st>>>(c->dm, c->tiles, c->prim, c->rec, a, e, pr,
                                               derr(c, 0, 2), derr(c, 0, 3));
@@ -1067,7 +1070,7 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
       }
       if (fused)
         HIPCHK(c, launch(c, KID_FACE_UPDATE_MIXED, [&] {
-          k_face_update_mixed<<<blocks_for(N), kTile, lds, c->stream>>>(
+          k_face_update_mixed<<<tiles_for(N), kTile, lds, c->stream>>>(
               c->dm, c->mt, c->tiles, q, c->recM, a, dparams_walk(c, 1),
               derr(c, s, 2 + 3 * stage), derr(c, s, 3 + 3 * stage));
         }));
@@ -1078,7 +1081,7 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
                                                                   derr(c, s, 3 + 3 * stage));
         }));
     }
-    HIPCHK(c, run_reduce_min(c, nxt));
+    HIPCHK(c, run_reduce_min(c, nxt, fused));
   }
   // FP64 cells from the FP64 extensives
   UpdArgs u{};
@@ -1568,7 +1571,7 @@ int rg_set_mesh(rg_ctx* c, const rg_mesh* m) {
   HIPCHK(c, dalloc(c, &c->flux, F));
   HIPCHK(c, dalloc(c, &c->rec, N));
   HIPCHK(c, dalloc(c, &c->staging, 4 * static_cast<size_t>(std::max(N, F))));
-  HIPCHK(c, dalloc(c, &c->partials, blocks_for(std::max(N, F))));
+  HIPCHK(c, dalloc(c, &c->partials, std::max(blocks_for(std::max(N, F)), tiles_for(N))));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->prm.precision == RG_MIXED) {
     // Incidence words of the FP32 update.  Periodic copies of a boundary
@@ -1941,7 +1944,7 @@ int rg_advance_noalloc(rg_ctx* c, int32_t integrator, int32_t nsteps, double* dt
     return arg_error(c, "rg_advance_noalloc: FP64 contexts without instances only");
   if (nsteps > kScratchSteps)
     return arg_error(c, "rg_advance_noalloc: more than 65536 steps in one call");
-  if (blocks_for(c->N) > static_cast<unsigned>(kScratchBlocks))
+  if (std::max(blocks_for(c->N), tiles_for(c->N)) > static_cast<unsigned>(kScratchBlocks))
     return arg_error(c, "rg_advance_noalloc: more than 16777216 cells");
   if (!c->scratch) {
     c->last = rg_error{};
@@ -2002,7 +2005,8 @@ int rg_adjoint_forward(rg_ctx* c, int32_t nsteps, int32_t every, double* dt_out)
     }
     HIPCHK(c, get(&c->rflux, static_cast<size_t>(F)));
     HIPCHK(c, get(&c->rwords, 4));
-    HIPCHK(c, get(&c->rpartials, static_cast<size_t>(blocks_for(std::max(N, F)))));
+    HIPCHK(c, get(&c->rpartials,
+                  static_cast<size_t>(std::max(blocks_for(std::max(N, F)), tiles_for(N)))));
     HIPCHK(c, get(&c->gP, n));
     HIPCHK(c, get(&c->gX, n));
     HIPCHK(c, get(&c->ga, n));
