@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(kTile) k_face_update_ens(DMesh m, DTiles tl,
     face_states(m, q, rec, sd.a, sd.b, fr, lf, rt);
     const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
     // modular_flux_calculator.cpp:56-67 (an ensemble has no face velocities)
-    const double speed = (fr.nx * 0.0 + fr.ny * 0.0) / an;
+    const double speed = zero_face_speed(fr.nx, fr.ny, an);
     Cons c;
     const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
     D4 o{0, 0, 0, 0};

@@ -717,8 +717,8 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
   }
   const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
   // modular_flux_calculator.cpp:56-67
-  const double wx = m.wx ? m.wx[f] : 0.0, wy = m.wy ? m.wy[f] : 0.0;
-  const double speed = (fr.nx * wx + fr.ny * wy) / an;
+  const double speed = m.wx ? (fr.nx * m.wx[f] + fr.ny * m.wy[f]) / an
+                            : zero_face_speed(fr.nx, fr.ny, an);
   Cons c;
   const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
   D4 o{0, 0, 0, 0};
@@ -1164,13 +1164,13 @@ __global__ void __launch_bounds__(kTile) k_face_update(DMesh m, DTiles tl, const
     face_states(m, q, rec, sd.a, sd.b, fr, lf, rt);
     const double an = vabs(fr.nx, fr.ny), ap = vabs(fr.px, fr.py);
     // modular_flux_calculator.cpp:56-67
-    double wx = 0.0, wy = 0.0;
+    double speed;
     if (m.wx) {
       const int f = tl.tface[e];
-      wx = m.wx[f];
-      wy = m.wy[f];
+      speed = (fr.nx * m.wx[f] + fr.ny * m.wy[f]) / an;
+    } else {
+      speed = zero_face_speed(fr.nx, fr.ny, an);
     }
-    const double speed = (fr.nx * wx + fr.ny * wy) / an;
     Cons c;
     const int rc = hllc_face(lf, rt, prm.gamma, fr.nx, fr.ny, an, fr.px, fr.py, ap, speed, c);
     D4 o{0, 0, 0, 0};

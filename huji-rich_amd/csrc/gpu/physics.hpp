@@ -43,6 +43,16 @@ __host__ __device__ __forceinline__ bool is_nan_ref(double x) {
   return 1 != b1 + b2;
 }
 
+// The face speed along the normal, (n . w) / |n| (modular_flux_calculator.cpp
+// :56-67, simple_flux_calculator.cpp:112-114), for w = 0 (no face
+// velocities): the numerator is a signed zero (or NaN), and a signed zero
+// divided by a positive |n| (finite or not) is that zero, so the quotient
+// needs no division; |n| = 0 or NaN keeps it.  Bit for bit the quotient.
+__host__ __device__ __forceinline__ double zero_face_speed(double nx, double ny, double an) {
+  const double num = nx * 0.0 + ny * 0.0;
+  return an > 0.0 ? num : num / an;
+}
+
 // IdealGas::dp2e / dp2c (ideal_gas.cpp:13-16, 40-44)
 __host__ __device__ __forceinline__ double dp2e(double g, double d, double p) { return p / d / (g - 1); }
 __host__ __device__ __forceinline__ double dp2c(double g, double d, double p) { return sqrt(g * p / d); }

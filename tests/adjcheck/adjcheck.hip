@@ -7,6 +7,9 @@ using namespace rg;
 
 extern "C" {
 
+// the face speed for zero face velocities, as the face kernels form it
+double ac_zero_face_speed(double nx, double ny, double an) { return zero_face_speed(nx, ny, an); }
+
 // frame = {nx, ny, an, px, py, ap}; cells (d, p, vx, vy); flux (m, mx, my, E)
 int ac_hllc(const double* L, const double* R, double g, const double* fr, double* out) {
   Cons c;

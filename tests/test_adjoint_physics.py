@@ -21,6 +21,8 @@ def lib():
     L = ctypes.CDLL(LIB)
     D = ctypes.POINTER(ctypes.c_double)
     L.ac_hllc.argtypes = [D, D, ctypes.c_double, D, D]
+    L.ac_zero_face_speed.argtypes = [ctypes.c_double] * 3
+    L.ac_zero_face_speed.restype = ctypes.c_double
     L.ac_hllc_vjp.argtypes = [D, D, ctypes.c_double, D, D, D, D]
     L.ac_recover_vjp.argtypes = [ctypes.c_double, ctypes.c_double, D, D, D]
     L.ac_init_vjp.argtypes = [ctypes.c_double, ctypes.c_double, D, D, D]
@@ -162,3 +164,24 @@ def test_cfl_inv_vjp(lib, moving):
         _, pc = _d(c)
         lib.ac_cfl_inv_vjp(G, width, pc, 1.0, cb.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         assert abs(fd - cb @ dc) <= 1e-7 * max(1.0, abs(fd))
+
+
+def test_zero_face_speed_is_the_quotient(lib):
+    """The face kernels' face speed without face velocities skips the
+    division where |n| > 0: bit for bit (n . 0) / |n| on signed zeros,
+    infinities, NaNs, denormals and ordinary values."""
+    vals = [0.0, -0.0, 5e-324, -5e-324, 1e-300, -1.0, 1.0, 3.7, -1e300, np.inf, -np.inf, np.nan]
+    ans = [0.0, -0.0, 5e-324, 1e-300, 0.5, 1.0, 1e300, np.inf, -np.inf, -1.0, np.nan]
+    n = 0
+    with np.errstate(all="ignore"):
+        for nx in vals:
+            for ny in vals:
+                for an in ans:
+                    want = (np.float64(nx) * 0.0 + np.float64(ny) * 0.0) / np.float64(an)
+                    got = lib.ac_zero_face_speed(nx, ny, an)
+                    if np.isnan(want):
+                        assert np.isnan(got), (nx, ny, an)
+                    else:
+                        assert np.float64(got).tobytes() == want.tobytes(), (nx, ny, an, got, want)
+                    n += 1
+    assert n == len(vals) ** 2 * len(ans)
