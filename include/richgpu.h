@@ -304,9 +304,10 @@ int32_t rg_tiles(rg_ctx* ctx, int64_t* entries, int32_t* max_entries);
 /* on = 1 (the default): the second-order Heun loops run the fused tile
  * stages (built at their first call on a mesh); on = 0: they run the
  * three-kernel stages (slopes, faces into a face array, cell update), with
- * the same results bit for bit and no tile memory built.  No counterpart in
- * the reference: for callers short of device memory, and for tests that
- * compare the two forms. */
+ * the same results bit for bit, and build no tiles (tiles already built stay
+ * allocated until the next rg_set_mesh).  No counterpart in the reference:
+ * for callers short of device memory, and for tests that compare the two
+ * forms. */
 int rg_use_tiles(rg_ctx* ctx, int32_t on);
 
 #ifdef __cplusplus
