@@ -963,12 +963,13 @@ struct UpdArgs {
 // kernels only).  With kTileFlux the face terms dt*area*F come from the
 // tile's LDS array `tflux` through the per-slot words `eloc` (entry within
 // the tile | TILE_SECOND) instead of the face array a.flux.
-// The loads of a cell's update that do not depend on the face sum (kEarly):
-// extensives, Heun operand, volume, width, error word and, for the tile
-// form, the first CH_UPD slot words.
 // Slots per chunk of the tile form's update (4 measured slower: 200.8 ->
 // 206.5 us on a 2M-cell mesh, at the same register count).
 constexpr int CH_TILE = CH_UPD;
+
+// The loads of a cell's update that do not depend on the face sum (kEarly):
+// extensives, Heun operand, volume, width, error word and, for the tile
+// form, the first CH_TILE slot words.
 struct UpdLoads {
   D4 x, o;
   double vol, wid;
