@@ -759,8 +759,16 @@ hipError_t tile_upload(rg_ctx* c, T** p, const std::vector<T>& v) {
 // (second order): per entry the face's id, sides and frame (FP64) or FP32
 // face record (mixed, whose slots read a periodic boundary face through its
 // lower-numbered copy).  A mesh whose largest tile would hold more than
-// kTileMaxEntries faces keeps the three-kernel stage (tiles_ok false).
+// kTileMaxEntries faces keeps the three-kernel stage (tiles_ok false), and so
+// does one whose tiles hold more than kTileDupNum / kTileDupDen entries per
+// face: the fused stage solves a face once per tile that reads it, and
+// where cell ids have little spatial locality the repeated faces and the
+// scattered reads cost more than the three-kernel stage's flux round trip.
+// On a 2048 x 1024 Kelvin-Helmholtz lattice with part of its cells shuffled
+// the two forms took the same time at about 1.39 entries per face (FP64)
+// and 1.35 (mixed); the Hilbert-ordered lattice has 1.08.
 // The fused stages run: allowed (rg_use_tiles) and the mesh's tiles fit.
+constexpr int64_t kTileDupNum = 13, kTileDupDen = 10;
 bool fused_ok(const rg_ctx* c) { return c->tiles_on && c->tiles_ok; }
 
 hipError_t build_tiles(rg_ctx* c) {
@@ -777,6 +785,7 @@ hipError_t build_tiles(rg_ctx* c) {
                         : tile_lists(s, N, c->dm_ne, [&](int k) { return s.inc[k].info; }, L);
   if (!ok) return hipSuccess;
   const int total = L.tofs.back();
+  if (kTileDupDen * total > kTileDupNum * static_cast<int64_t>(c->F)) return hipSuccess;
   std::vector<FaceSides> tside(L.tface.size());
   rmp::parallel_for(static_cast<int64_t>(L.tface.size()), [&](int64_t e) {
     const int f = L.tface[static_cast<size_t>(e)];

@@ -171,3 +171,32 @@ def test_fused_loop_with_face_velocities_equals_the_oracle():
     np.testing.assert_array_equal(runs[0][2], eo)
     for a, b in zip(*runs):
         np.testing.assert_array_equal(a, b)
+
+
+def test_shuffled_cell_order_keeps_the_three_kernel_stage():
+    """Tiles of cells with little spatial locality read many faces twice; the
+    fused stage is then slower than the three-kernel one, and a mesh whose
+    tiles would hold more than 1.3 entries per face keeps the latter.  The
+    same points in lattice order and shuffled: the first runs fused, the
+    second not, and both equal the oracle bit for bit."""
+    from rich_amd.gpu import RichGPU
+    box = (0.0, 1.0, 0.0, 0.75)
+    m0 = rmesh.lattice(48, 36, box, jitter=2e-2, seed=7)
+    N = m0.n_cells
+    order = np.random.default_rng(4).permutation(N)
+    for ids, fused in ((np.arange(N), True), (order, False)):
+        m = rmesh.points(np.asarray(m0.pt_x[:N])[ids], np.asarray(m0.pt_y[:N])[ids], box)
+        n, _ = _entries(m)
+        assert (10 * n <= 13 * m.n_faces) == fused
+        cells = _rough(m, seed=6)
+        g = RichGPU(m, order=2)
+        g.set_cells(cells)
+        g.init_extensives()
+        dts = g.advance(3, integrator=2)
+        assert g.tiles()[0] == fused
+        o = po.Oracle(m, order=2)
+        co, eo, dto = o.run(cells, o.init_extensives(cells), 3, integrator=2)
+        assert_parity(dts, dto, what="dt")
+        np.testing.assert_array_equal(g.get_cells(), co)
+        np.testing.assert_array_equal(g.get_extensives(), eo)
+        g.close()
