@@ -296,7 +296,10 @@ int rg_profile_read(rg_ctx* ctx, int32_t kernel_id, int64_t* launches, double* t
 int64_t rg_device_bytes(rg_ctx* ctx);
 /* Whether the second-order Heun loops (rg_advance, the ensemble graph, the
  * adjoint) run each stage's faces and cell update fused per tile of 256
- * cells: 1 once the mesh's tiles are built and fit, else 0.  *entries and
+ * cells: 1 once the mesh's tiles are built, fit the kernel's shared memory
+ * and hold at most 1.3 entries per face (cells numbered along a
+ * space-filling curve give about 1.08; in random order about 2, where the
+ * three-kernel stages are faster), else 0.  *entries and
  * *max_entries (either may be NULL) receive the faces solved per stage (a
  * face between two tiles counts twice) and the largest tile's count.  No
  * counterpart in the reference: a diagnostic of this library's layout. */
