@@ -31,6 +31,19 @@
 
 #include "physics.hpp"
 
+// The library is built from two translation units: richgpu.hip, under the
+// max-ILP instruction scheduler, and default_sched.hip, under the
+// compiler's default one (the FP64 slope kernel and the mixed-precision
+// stage kernels run faster there).  Each kernel of these headers is defined
+// by one unit: RG_GLOBAL kernels by richgpu.hip, RG_GLOBAL_DEFAULT_SCHED
+// ones by default_sched.hip.  In the other unit a kernel is a template that
+// nothing instantiates, so that unit emits no code for it and the two link
+// into one library.
+#ifndef RG_GLOBAL
+#define RG_GLOBAL __global__
+#define RG_GLOBAL_DEFAULT_SCHED template <int = 0> __global__
+#endif
+
 namespace rg {
 
 constexpr int BC_RIGID = 0;
@@ -254,7 +267,7 @@ __device__ __forceinline__ void block_min(double v, double* partials) {
 }
 
 // One block: min of n partials into *slot (bits of a non-negative double).
-__global__ void __launch_bounds__(1024) k_reduce_min(const double* __restrict__ partials, int n,
+RG_GLOBAL void __launch_bounds__(1024) k_reduce_min(const double* __restrict__ partials, int n,
                                                      unsigned long long* slot) {
   __shared__ double wmin[16];
   double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
@@ -273,7 +286,7 @@ __global__ void __launch_bounds__(1024) k_reduce_min(const double* __restrict__ 
 // RiemannSolver on n pairs of face-frame Primitives (SoA [6][n]: density,
 // pressure, x and y velocity, energy, sound speed), face velocities w (null:
 // zero), fluxes out [4][n] (mass, energy, x and y momentum).
-__global__ void __launch_bounds__(256) k_riemann(const double* __restrict__ l,
+RG_GLOBAL void __launch_bounds__(256) k_riemann(const double* __restrict__ l,
                                                  const double* __restrict__ r,
                                                  const double* __restrict__ w,
                                                  double* __restrict__ out, int n, DErr err) {
@@ -293,7 +306,7 @@ __global__ void __launch_bounds__(256) k_riemann(const double* __restrict__ l,
 
 // ------------------------------------------------------------------ mesh --
 // Per-face static tables.
-__global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceFrame* fframe,
+RG_GLOBAL void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceFrame* fframe,
                                                    double* area) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
@@ -330,7 +343,7 @@ __global__ void __launch_bounds__(256) k_geo_faces(DMesh m, FaceRec* frec, FaceF
 // Per-cell static table: the geometric half of calc_naive_slope
 // (LinearGaussImproved.cpp:67-134: matrix, singularity test, inverse) or of
 // GetBoundarySlope (:401-422), and GetWidth (VoronoiMesh.cpp:410-413).
-__global__ void __launch_bounds__(256) k_geo_cells(DMesh m, CellGeo* cg, double* width) {
+RG_GLOBAL void __launch_bounds__(256) k_geo_cells(DMesh m, CellGeo* cg, double* width) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const int b = m.off[i], e = m.off[i + 1], n = e - b;
@@ -442,7 +455,7 @@ constexpr int CH_UPD = 8;
 // copies of the per-incidence geometry -- the same values the slope and
 // update loops would read through the face index, so the results do not
 // change.
-__global__ void __launch_bounds__(256) k_geo_ell(DMesh m, int* __restrict__ enbr,
+RG_GLOBAL void __launch_bounds__(256) k_geo_ell(DMesh m, int* __restrict__ enbr,
                                                  unsigned* __restrict__ einfo, D2* __restrict__ eh,
                                                  D2* __restrict__ eo, double* __restrict__ earea) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -617,15 +630,19 @@ __device__ __forceinline__ void slopes_cell(const DMesh& m, const D4* __restrict
 }
 
 // Lane l of wavefront w takes the cell of ELL column l, so that the lanes
-// that run out of faces first are the last ones.
-__global__ void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
-                                                Rec* __restrict__ out, DParams prm, DErr err) {
+// that run out of faces first are the last ones.  Built by
+// default_sched.hip; launch_slopes enqueues it.
+RG_GLOBAL_DEFAULT_SCHED void __launch_bounds__(256) k_slopes(DMesh m, const D4* __restrict__ q,
+                                                             Rec* __restrict__ out, DParams prm,
+                                                             DErr err) {
   const int t = walk_block(prm) * blockDim.x + threadIdx.x;
   if (t >= m.N) return;
   slopes_cell(m, q, out, prm, err, (t & ~63) + m.lanecell[t]);
 }
+hipError_t launch_slopes(int blocks, int threads, hipStream_t st, DMesh m, const D4* q, Rec* out,
+                         DParams prm, DErr err);
 // Persistent form: a fixed grid walks the cells in tiles of blockDim.x.
-__global__ void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_slopes_p(DMesh m, const D4* __restrict__ q,
                                                   Rec* __restrict__ out, DParams prm, DErr err) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.N; i += gridDim.x * blockDim.x)
     slopes_cell(m, q, out, prm, err, i);
@@ -734,7 +751,7 @@ __device__ __forceinline__ void flux2_face(const DMesh& m, const D4* __restrict_
 }
 
 // dt_slot non-null: write dt*area*F with dt = cfl * *dt_slot (the step loop)
-__global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q,
                                                const Rec* __restrict__ rec, D4* __restrict__ flux,
                                                DParams prm, DErr err,
                                                const unsigned long long* __restrict__ dt_slot) {
@@ -743,7 +760,7 @@ __global__ void __launch_bounds__(256) k_flux2(DMesh m, const D4* __restrict__ q
   flux2_face(m, q, rec, flux, prm, err, walk_block(prm) * blockDim.x + threadIdx.x, dt);
 }
 // ModularFluxCalculator over PCM; dt_slot as for k_flux2
-__global__ void __launch_bounds__(256) k_flux_pcm(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_flux_pcm(DMesh m, const D4* __restrict__ q,
                                                   D4* __restrict__ flux, DParams prm, DErr err,
                                                   const unsigned long long* __restrict__ dt_slot) {
   const double dt =
@@ -751,7 +768,7 @@ __global__ void __launch_bounds__(256) k_flux_pcm(DMesh m, const D4* __restrict_
   flux2_face<true>(m, q, nullptr, flux, prm, err, blockIdx.x * blockDim.x + threadIdx.x, dt);
 }
 // Persistent form: a fixed grid walks the faces in tiles of blockDim.x.
-__global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__ q,
                                                  const Rec* __restrict__ rec,
                                                  D4* __restrict__ flux, DParams prm, DErr err) {
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < m.F; f += gridDim.x * blockDim.x)
@@ -759,7 +776,7 @@ __global__ void __launch_bounds__(256) k_flux2_p(DMesh m, const D4* __restrict__
 }
 
 // SpatialReconstruction output res[f].first / .second
-__global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restrict__ q,
                                                      const Rec* __restrict__ rec, D4* left,
                                                      D4* right) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -775,7 +792,7 @@ __global__ void __launch_bounds__(256) k_face_states(DMesh m, const D4* __restri
 
 // PCM::operator() (interpolations/PCM.cpp:5-37): res[f].first / .second are
 // the cells on the two sides, ghost sides as the ghost generator fills them
-__global__ void __launch_bounds__(256) k_face_states_pcm(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_face_states_pcm(DMesh m, const D4* __restrict__ q,
                                                          D4* left, D4* right) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
@@ -824,7 +841,7 @@ __device__ __forceinline__ void simple_face_states(const DMesh& m, const D4* __r
 }
 
 // SimpleFluxCalculator body (simple_flux_calculator.cpp:80-126,167-189).
-__global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q,
                                                D4* __restrict__ flux, DParams prm, DErr err) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
@@ -849,7 +866,7 @@ __global__ void __launch_bounds__(256) k_flux1(DMesh m, const D4* __restrict__ q
 // flux call that failed (cells q and, at second order, their slopes rec):
 // the states Hllc received (rotated into the face frame, then shifted by
 // the face speed), the face speed and the three wave speeds.  One thread.
-__global__ void k_face_entries(DMesh m, const D4* __restrict__ q, const Rec* __restrict__ rec,
+RG_GLOBAL void k_face_entries(DMesh m, const D4* __restrict__ q, const Rec* __restrict__ rec,
                                int order, double gamma, int f, double* __restrict__ out) {
   if (blockIdx.x != 0 || threadIdx.x != 0 || f < 0 || f >= m.F) return;
   Cell L, R;
@@ -913,7 +930,7 @@ __device__ __forceinline__ double cfl_inv(const DMesh& m, int i, double gamma, d
   return 1.0 / res;
 }
 
-__global__ void __launch_bounds__(256) k_cfl(DMesh m, const D4* __restrict__ q, double gamma,
+RG_GLOBAL void __launch_bounds__(256) k_cfl(DMesh m, const D4* __restrict__ q, double gamma,
                                              double* partials) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v = __longlong_as_double(static_cast<long long>(DBL_INF_BITS));
@@ -921,7 +938,7 @@ __global__ void __launch_bounds__(256) k_cfl(DMesh m, const D4* __restrict__ q, 
   block_min(v, partials);
 }
 
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
+RG_GLOBAL void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -1118,7 +1135,7 @@ __device__ __forceinline__ double update_cell(const DMesh& m, const UpdArgs& a,
   return cfl_v;
 }
 
-__global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm, DErr err) {
+RG_GLOBAL void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm, DErr err) {
   const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   const double dt = a.dt_slot ? prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot))
                               : a.dt;
@@ -1139,7 +1156,7 @@ __global__ void __launch_bounds__(256) k_update(DMesh m, UpdArgs a, DParams prm,
 // is computed by both tiles, with equal results.  The stage must not write
 // what its faces read: the update's outputs (a.out, a.cells) are other
 // buffers than the cells q and slopes rec.
-__global__ void __launch_bounds__(kTile) k_face_update(DMesh m, DTiles tl, const D4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(kTile) k_face_update(DMesh m, DTiles tl, const D4* __restrict__ q,
                                                      const Rec* __restrict__ rec, UpdArgs a,
                                                      DParams prm, DErr errf, DErr erru) {
   extern __shared__ D4 tflux[];
@@ -1198,7 +1215,7 @@ __global__ void __launch_bounds__(kTile) k_face_update(DMesh m, DTiles tl, const
 }
 
 // init_extensives (hdsim2d.cpp:41-66)
-__global__ void __launch_bounds__(256) k_init_ext(DMesh m, const D4* __restrict__ q, D4* x,
+RG_GLOBAL void __launch_bounds__(256) k_init_ext(DMesh m, const D4* __restrict__ q, D4* x,
                                                   double gamma) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
@@ -1210,11 +1227,11 @@ __global__ void __launch_bounds__(256) k_init_ext(DMesh m, const D4* __restrict_
 }
 
 // SoA <-> AoS for the host API
-__global__ void k_pack4(const double* __restrict__ s, D4* __restrict__ d, int n) {
+RG_GLOBAL void k_pack4(const double* __restrict__ s, D4* __restrict__ d, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = D4{s[i], s[n + i], s[2 * static_cast<size_t>(n) + i], s[3 * static_cast<size_t>(n) + i]};
 }
-__global__ void k_unpack4(const D4* __restrict__ s, double* __restrict__ d, int n) {
+RG_GLOBAL void k_unpack4(const D4* __restrict__ s, double* __restrict__ d, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const D4 v = s[i];

@@ -157,7 +157,7 @@ __device__ __forceinline__ void shocked_fieldf(float dphi, float cval, float cel
 }
 
 // Static FP32 tables, formed in FP64 and rounded once.
-__global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
+RG_GLOBAL void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= m.F) return;
   const FaceRec r = m.frec[f];
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_geo_mixed_faces(DMesh m, FaceM* face) {
 }
 
 // ELL geometry of every incidence: one thread per cell over its faces.
-__global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, float2* __restrict__ eh,
+RG_GLOBAL void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, float2* __restrict__ eh,
                                                        float2* __restrict__ eo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(256) k_geo_mixed_ell(DMesh m, float2* __restri
   }
 }
 
-__global__ void __launch_bounds__(256) k_geo_mixed_cells(DMesh m, CellGeoM* cg) {
+RG_GLOBAL void __launch_bounds__(256) k_geo_mixed_cells(DMesh m, CellGeoM* cg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const CellGeo g = m.cgeo[i];
@@ -203,9 +203,10 @@ __global__ void __launch_bounds__(256) k_geo_mixed_cells(DMesh m, CellGeoM* cg) 
 constexpr int CH_M = 8;
 
 // LinearGaussImproved in FP32.
-__global__ void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t, const F4* __restrict__ q,
-                                                      RecM* __restrict__ out, DParams prm,
-                                                      DErr err) {
+RG_GLOBAL_DEFAULT_SCHED void __launch_bounds__(256) k_slopes_mixed(DMesh m, MTables t,
+                                                                   const F4* __restrict__ q,
+                                                                   RecM* __restrict__ out,
+                                                                   DParams prm, DErr err) {
   const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   if (i >= m.N) return;
   const int b = m.off[i], e = m.off[i + 1], deg = e - b;
@@ -418,7 +419,7 @@ __device__ __forceinline__ float face_speed_m(const FaceM& fr, float wx, float w
   return fr.nx * wx + fr.ny * wy;
 }
 
-__global__ void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4* __restrict__ q,
+RG_GLOBAL void __launch_bounds__(256) k_flux_mixed(DMesh m, MTables t, const F4* __restrict__ q,
                                                     const RecM* __restrict__ rec,
                                                     F4* __restrict__ flux, DParams prm, DErr err) {
   const int f = walk_block(prm) * blockDim.x + threadIdx.x;
@@ -466,7 +467,7 @@ struct UpdArgsM {
   int flags;
 };
 
-__global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArgsM a, DParams prm,
+RG_GLOBAL void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArgsM a, DParams prm,
                                                       DErr err) {
   const int i = walk_block(prm) * blockDim.x + threadIdx.x;
   const double dt = prm.cfl * __longlong_as_double(static_cast<long long>(*a.dt_slot));
@@ -528,11 +529,9 @@ __global__ void __launch_bounds__(256) k_update_mixed(DMesh m, MTables t, UpdArg
 // k_update_mixed reads it) into LDS, then k_update_mixed's update of the
 // tile's cells from those fluxes.  The update's outputs are other buffers
 // than the cells and slopes the faces read.
-__global__ void __launch_bounds__(kTile) k_face_update_mixed(DMesh m, MTables t, DTiles tl,
-                                                           const F4* __restrict__ q,
-                                                           const RecM* __restrict__ rec,
-                                                           UpdArgsM a, DParams prm, DErr errf,
-                                                           DErr erru) {
+RG_GLOBAL_DEFAULT_SCHED void __launch_bounds__(kTile) k_face_update_mixed(
+    DMesh m, MTables t, DTiles tl, const F4* __restrict__ q, const RecM* __restrict__ rec,
+    UpdArgsM a, DParams prm, DErr errf, DErr erru) {
   extern __shared__ F4 tfluxm[];
   // slots per chunk of the update: the fluxes come from LDS, so a short
   // chunk hides their latency and keeps the registers down
@@ -641,7 +640,7 @@ __global__ void __launch_bounds__(kTile) k_face_update_mixed(DMesh m, MTables t,
 
 // The tiles' per-entry face data in entry order (set-up): the FP64 frame
 // or the FP32 face record, whichever is requested.
-__global__ void __launch_bounds__(256) k_tile_frames(DMesh m, MTables t,
+RG_GLOBAL void __launch_bounds__(256) k_tile_frames(DMesh m, MTables t,
                                                      const int* __restrict__ tface,
                                                      FaceFrame* __restrict__ tframe,
                                                      FaceM* __restrict__ tfm, int n) {
@@ -652,7 +651,7 @@ __global__ void __launch_bounds__(256) k_tile_frames(DMesh m, MTables t,
   if (tfm) tfm[e] = t.face[f];
 }
 
-__global__ void k_to_f32(const D4* __restrict__ s, F4* __restrict__ d, int n) {
+RG_GLOBAL void k_to_f32(const D4* __restrict__ s, F4* __restrict__ d, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const D4 v = s[i];
@@ -660,5 +659,12 @@ __global__ void k_to_f32(const D4* __restrict__ s, F4* __restrict__ d, int n) {
               static_cast<float>(v.d)};
   }
 }
+
+// Enqueue the kernels default_sched.hip builds.
+hipError_t launch_slopes_mixed(int blocks, int threads, hipStream_t st, DMesh m, MTables t,
+                               const F4* q, RecM* out, DParams prm, DErr err);
+hipError_t launch_face_update_mixed(int blocks, int threads, size_t lds, hipStream_t st, DMesh m,
+                                    MTables t, DTiles tl, const F4* q, const RecM* rec,
+                                    UpdArgsM a, DParams prm, DErr errf, DErr erru);
 
 }  // namespace rg

@@ -530,8 +530,8 @@ bool ready(const rg_ctx* c, bool need_cells, bool need_ext) {
 hipError_t run_slopes(rg_ctx* c, const D4* cells, int step, int phase, Rec* out, int rev) {
   Rec* r = out ? out : c->rec;
   return launch(c, KID_SLOPES, [&] {
-    k_slopes<<<blocks_for(c->N), kBlock, 0, c->stream>>>(c->dm, cells, r, dparams_walk(c, rev),
-                                                         derr(c, step, phase));
+    (void)launch_slopes(blocks_for(c->N), kBlock, c->stream, c->dm, cells, r, dparams_walk(c, rev),
+                        derr(c, step, phase));
   });
 }
 
@@ -1048,9 +1048,8 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
       // the FP64 step (heun_step)
       const F4* q = stage == 0 ? c->primM : c->mid_primM;
       HIPCHK(c, launch(c, KID_SLOPES_MIXED, [&] {
-        k_slopes_mixed<<<blocks_for(N), kBlock, 0, c->stream>>>(
-            c->dm, c->mt, q, c->recM, dparams_walk(c, fused ? 0 : stage),
-            derr(c, s, 1 + 3 * stage));
+        (void)launch_slopes_mixed(blocks_for(N), kBlock, c->stream, c->dm, c->mt, q, c->recM,
+                                  dparams_walk(c, fused ? 0 : stage), derr(c, s, 1 + 3 * stage));
       }));
       if (!fused)
         HIPCHK(c, launch(c, KID_FLUX_MIXED, [&] {
@@ -1079,9 +1078,9 @@ int advance_mixed(rg_ctx* c, int nsteps, double* dt_out) {
       }
       if (fused)
         HIPCHK(c, launch(c, KID_FACE_UPDATE_MIXED, [&] {
-          k_face_update_mixed<<<tiles_for(N), kTile, lds, c->stream>>>(
-              c->dm, c->mt, c->tiles, q, c->recM, a, dparams_walk(c, 1),
-              derr(c, s, 2 + 3 * stage), derr(c, s, 3 + 3 * stage));
+          (void)launch_face_update_mixed(tiles_for(N), kTile, lds, c->stream, c->dm, c->mt,
+                                         c->tiles, q, c->recM, a, dparams_walk(c, 1),
+                                         derr(c, s, 2 + 3 * stage), derr(c, s, 3 + 3 * stage));
         }));
       else
         HIPCHK(c, launch(c, KID_UPDATE_MIXED, [&] {
